@@ -1,0 +1,210 @@
+"""ctypes binding of libflagsparse_hip.so (the C ABI in include/flagsparse_hip.h).
+
+This module is the only place that touches the shared library.  It maps
+torch dtypes to the ABI's value/index codes, packs device pointers into the
+``fs_csr`` / ``fs_plan`` structs, passes torch's current HIP stream, and turns
+``FS_ERR_*`` statuses into the Python exceptions the reference raises
+(ValueError / TypeError / IndexError / RuntimeError).
+
+There is no fallback: if the library is missing or cannot be loaded, every
+entry point raises ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "libflagsparse_hip.so")
+
+FS_OK, FS_ERR_VALUE, FS_ERR_TYPE, FS_ERR_INDEX, FS_ERR_HIP, FS_ERR_UNSUPPORTED = range(6)
+FS_F16, FS_BF16, FS_F32, FS_F64, FS_C32, FS_C64, FS_C128 = range(7)
+FS_I32, FS_I64 = 0, 1
+FS_ACC_NATIVE, FS_ACC_WIDE = 0, 1
+FS_STATS_LEN = 16
+
+_COMPLEX32 = getattr(torch, "complex32", None) or getattr(torch, "chalf", None)
+
+VALUE_CODE = {
+    torch.float16: FS_F16,
+    torch.bfloat16: FS_BF16,
+    torch.float32: FS_F32,
+    torch.float64: FS_F64,
+    torch.complex64: FS_C64,
+    torch.complex128: FS_C128,
+}
+if _COMPLEX32 is not None:
+    VALUE_CODE[_COMPLEX32] = FS_C32
+INDEX_CODE = {torch.int32: FS_I32, torch.int64: FS_I64}
+
+
+class FsCsr(ctypes.Structure):
+    _fields_ = [
+        ("value_type", ctypes.c_int32),
+        ("index_type", ctypes.c_int32),
+        ("ptr_type", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("n_rows", ctypes.c_int64),
+        ("n_cols", ctypes.c_int64),
+        ("nnz", ctypes.c_int64),
+        ("row_ptr", ctypes.c_void_p),
+        ("col_idx", ctypes.c_void_p),
+        ("values", ctypes.c_void_p),
+    ]
+
+
+class FsPlan(ctypes.Structure):
+    _fields_ = [
+        ("accumulate", ctypes.c_int32),
+        ("group_lanes", ctypes.c_int32),
+        ("xcd_remap", ctypes.c_int32),
+        ("unroll", ctypes.c_int32),
+        ("short_max", ctypes.c_int64),
+        ("chunk_nnz", ctypes.c_int64),
+        ("n_long", ctypes.c_int64),
+        ("long_rows", ctypes.c_void_p),
+        ("long_chunk_ptr", ctypes.c_void_p),
+        ("n_chunks", ctypes.c_int64),
+        ("chunk_long", ctypes.c_void_p),
+        ("counters", ctypes.c_void_p),
+        ("partials", ctypes.c_void_p),
+        ("partials_bytes", ctypes.c_int64),
+    ]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+# Every symbol the header declares, with its ctypes signature.
+_SIGNATURES = {
+    "fs_partials_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64]),
+    "fs_spmv_csr": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_double, ctypes.c_double, ctypes.POINTER(FsPlan),
+                                   ctypes.c_void_p]),
+    "fs_spmm_csr": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_int64,
+                                   ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                   ctypes.c_double, ctypes.c_double, ctypes.POINTER(FsPlan),
+                                   ctypes.c_int32, ctypes.c_void_p]),
+    "fs_csr_analyze": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_void_p]),
+    "fs_last_error": (ctypes.c_char_p, []),
+    "fs_version": (ctypes.c_char_p, []),
+}
+
+
+def load():
+    """Load the shared library (after torch, so torch's HIP runtime is reused)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m flagsparse_public_amd.build`")
+        try:
+            lib = ctypes.CDLL(LIB_PATH)
+        except OSError as exc:
+            raise RuntimeError(f"cannot load {LIB_PATH}: {exc}") from exc
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return tuple(_SIGNATURES)
+
+
+def check(status: int, what: str = "") -> None:
+    if status == FS_OK:
+        return
+    msg = load().fs_last_error().decode("utf-8", "replace")
+    if what and not msg.startswith(what):
+        msg = f"{what}: {msg}"
+    if status == FS_ERR_VALUE:
+        raise ValueError(msg)
+    if status == FS_ERR_TYPE:
+        raise TypeError(msg)
+    if status == FS_ERR_INDEX:
+        raise IndexError(msg)
+    raise RuntimeError(msg)
+
+
+def value_code(dtype) -> int:
+    try:
+        return VALUE_CODE[dtype]
+    except KeyError:
+        raise TypeError(f"unsupported value dtype {dtype}") from None
+
+
+def index_code(dtype) -> int:
+    try:
+        return INDEX_CODE[dtype]
+    except KeyError:
+        raise TypeError(f"unsupported index dtype {dtype}") from None
+
+
+def ptr(t) -> int | None:
+    if t is None or t.numel() == 0:
+        return None
+    return t.data_ptr()
+
+
+def make_csr(data, indices, indptr, n_rows, n_cols) -> FsCsr:
+    return FsCsr(
+        value_type=value_code(data.dtype),
+        index_type=index_code(indices.dtype),
+        ptr_type=index_code(indptr.dtype),
+        reserved=0,
+        n_rows=int(n_rows),
+        n_cols=int(n_cols),
+        nnz=int(data.numel()),
+        row_ptr=ptr(indptr),
+        col_idx=ptr(indices),
+        values=ptr(data),
+    )
+
+
+def stream_handle(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def analyze(csr: FsCsr, device) -> list[int]:
+    """Run fs_csr_analyze and copy the FS_STATS_LEN statistics to the host."""
+    lib = load()
+    stats = torch.empty(FS_STATS_LEN, dtype=torch.int64, device=device)
+    with torch.cuda.device(device):
+        check(lib.fs_csr_analyze(ctypes.byref(csr), stats.data_ptr(), stream_handle(device)),
+              "fs_csr_analyze")
+    return stats.cpu().tolist()
+
+
+def spmv(csr: FsCsr, x, y, alpha: float, beta: float, plan: FsPlan, device) -> None:
+    lib = load()
+    with torch.cuda.device(device):
+        check(lib.fs_spmv_csr(ctypes.byref(csr), ptr(x), ptr(y), float(alpha), float(beta),
+                              ctypes.byref(plan), stream_handle(device)), "fs_spmv_csr")
+
+
+def spmm(csr: FsCsr, B, ldb: int, C, ldc: int, n_dense: int, alpha: float, beta: float,
+         plan: FsPlan, col_tile: int, device) -> None:
+    lib = load()
+    with torch.cuda.device(device):
+        check(lib.fs_spmm_csr(ctypes.byref(csr), ptr(B), int(ldb), ptr(C), int(ldc),
+                              int(n_dense), float(alpha), float(beta), ctypes.byref(plan),
+                              int(col_tile), stream_handle(device)), "fs_spmm_csr")
+
+
+def partials_bytes(value_type: int, accumulate: int, n_chunks: int, n_dense: int = 1) -> int:
+    return int(load().fs_partials_bytes(value_type, accumulate, n_chunks, n_dense))
+
+
+def version() -> str:
+    return load().fs_version().decode()

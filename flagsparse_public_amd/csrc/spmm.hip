@@ -1,0 +1,370 @@
+// spmm.hip -- CSR SpMM for gfx950: C = alpha * A B + beta * C, row-major B and C.
+//
+// A group of G lanes owns one row and a tile of G*VW dense columns; each lane
+// owns VW consecutive columns and gathers them from the B row of every entry
+// with one vector load (VW * element size bytes, e.g. 16 B for fp32 x4), so a
+// group reads one contiguous B-row segment per entry.  U entries are kept in
+// flight per lane.  Rows longer than short_max are cut into chunks reduced by
+// whole wavefronts (64/G groups interleave over the chunk's entries and are
+// combined with xor exchanges), with the same last-arriver fix-up as SpMV:
+// the row's partials are summed in chunk order.
+//
+// Replaces the reference's ALG1 row x column-tile kernel (spmm_csr.py:92-134,
+// 525-599) and the bucketed opt / alg2 kernels (spmm_csr.py:602-919,
+// spmm_csr_opt_alg2.py:338-629).
+#include <type_traits>
+
+#include "fs_host.hpp"
+
+namespace fs {
+
+constexpr int kMmBlock = 256;
+constexpr int kMmWaves = kMmBlock / kWave;
+
+template <typename V, typename I, typename P>
+struct SpmmParams {
+  const P* ptr;
+  const I* idx;
+  const V* val;
+  const V* B;
+  V* C;
+  int64_t ldb, ldc, n_dense;
+  int64_t n_rows;
+  int64_t n_tiles;  // column tiles of G*VW columns
+  double alpha, beta;
+  int64_t short_max;
+  int64_t n_chunk_blocks;
+  int64_t n_chunks;
+  int64_t chunk_nnz;
+  const int64_t* long_rows;
+  const int64_t* long_chunk_ptr;
+  const int32_t* chunk_long;
+  int32_t* counters;  // [n_long * n_tiles]
+  void* partials;     // [n_chunks * n_dense] accumulators
+  int xcd_remap;
+};
+
+template <typename A> struct MmChunkAcc { using T = A; };
+template <> struct MmChunkAcc<Plain<float>> { using T = Comp; };
+
+// True xor exchange (lane ^ K): quad_perm for 1 and 2, ds_swizzle for 4..16,
+// ds_bpermute for 32.  Used to combine lane groups at equal positions.
+template <int K> __device__ __forceinline__ int xor_i32(int v) {
+  if constexpr (K == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+  else if constexpr (K == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+  else if constexpr (K == 4 || K == 8 || K == 16) return __builtin_amdgcn_ds_swizzle(v, 0x1F | (K << 10));
+  else {
+    static_assert(K == 32, "bad xor distance");
+    return __builtin_amdgcn_ds_bpermute((__lane_id() ^ 32) << 2, v);
+  }
+}
+template <int K> __device__ __forceinline__ float xor_x(float v) { return __int_as_float(xor_i32<K>(__float_as_int(v))); }
+template <int K> __device__ __forceinline__ double xor_x(double v) {
+  int64_t b = __double_as_longlong(v);
+  int lo = xor_i32<K>((int)(b & 0xffffffff));
+  int hi = xor_i32<K>((int)(b >> 32));
+  return __longlong_as_double(((int64_t)(uint32_t)lo) | ((int64_t)hi << 32));
+}
+template <int K, typename T> __device__ __forceinline__ cplx<T> xor_x(cplx<T> v) { return {xor_x<K>(v.re), xor_x<K>(v.im)}; }
+template <int K, typename T> __device__ __forceinline__ Plain<T> xor_x(const Plain<T>& a) { Plain<T> o; o.s = xor_x<K>(a.s); return o; }
+template <int K> __device__ __forceinline__ Comp xor_x(const Comp& a) { Comp o; o.s = xor_x<K>(a.s); o.c = xor_x<K>(a.c); return o; }
+
+// Sum accumulators over lane groups of width G inside a wavefront (positions
+// kept): after this every lane holds the sum over the 64/G groups.
+template <int G, typename A> __device__ __forceinline__ void cross_group_reduce(A& a) {
+  if constexpr (G <= 1) { A o = xor_x<1>(a); a.merge(o); }
+  if constexpr (G <= 2) { A o = xor_x<2>(a); a.merge(o); }
+  if constexpr (G <= 4) { A o = xor_x<4>(a); a.merge(o); }
+  if constexpr (G <= 8) { A o = xor_x<8>(a); a.merge(o); }
+  if constexpr (G <= 16) { A o = xor_x<16>(a); a.merge(o); }
+  if constexpr (G <= 32) { A o = xor_x<32>(a); a.merge(o); }
+}
+
+// Load VW consecutive stored values starting at element `off` as compute type.
+template <typename C, int VW, typename S>
+__device__ __forceinline__ void load_vec(C (&out)[VW], const S* __restrict__ p, int64_t off) {
+  if constexpr (std::is_same<S, float>::value && VW == 4) {
+    const float4 v = *reinterpret_cast<const float4*>(p + off);
+    out[0] = (C)v.x; out[1] = (C)v.y; out[2] = (C)v.z; out[3] = (C)v.w;
+  } else if constexpr (std::is_same<S, float>::value && VW == 2) {
+    const float2 v = *reinterpret_cast<const float2*>(p + off);
+    out[0] = (C)v.x; out[1] = (C)v.y;
+  } else if constexpr (std::is_same<S, double>::value && VW == 2) {
+    const double2 v = *reinterpret_cast<const double2*>(p + off);
+    out[0] = (C)v.x; out[1] = (C)v.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < VW; ++k) out[k] = load_as<C>(p, off + k);
+  }
+}
+
+template <typename A> __device__ __forceinline__ void store_partial_mm(A* p, const A& a) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(&a);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(p);
+#pragma unroll
+  for (int w = 0; w < (int)(sizeof(A) / 4); ++w)
+    __hip_atomic_store(dst + w, src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename A> __device__ __forceinline__ A load_partial_mm(const A* p) {
+  A a;
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&a);
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+  for (int w = 0; w < (int)(sizeof(A) / 4); ++w)
+    dst[w] = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return a;
+}
+
+// Accumulate entries j = s + first, s + first + step, ... < e into acc[VW] for
+// the lane's columns n0 .. n0+VW-1.
+template <int VW, int U, typename A, typename V, typename I>
+__device__ __forceinline__ void mm_accumulate(A (&acc)[VW], const I* __restrict__ idx,
+                                              const V* __restrict__ val,
+                                              const V* __restrict__ B, int64_t ldb,
+                                              int64_t n0, int64_t s, int64_t e, int64_t step) {
+  using C = typename A::compute_t;
+  int64_t j = s;
+  for (; j + (int64_t)(U - 1) * step < e; j += (int64_t)U * step) {
+    int64_t col[U];
+    C a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      col[u] = ldg_index(idx, j + u * step);
+      a[u] = load_as<C>(val, j + u * step);
+    }
+    C b[U][VW];
+#pragma unroll
+    for (int u = 0; u < U; ++u) load_vec<C, VW>(b[u], B, col[u] * ldb + n0);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < VW; ++k) acc[k].add_prod(a[u], b[u][k]);
+  }
+  for (; j < e; j += step) {
+    const int64_t c = ldg_index(idx, j);
+    const C a = load_as<C>(val, j);
+    C b[VW];
+    load_vec<C, VW>(b, B, c * ldb + n0);
+#pragma unroll
+    for (int k = 0; k < VW; ++k) acc[k].add_prod(a, b[k]);
+  }
+}
+
+template <typename V, typename A, typename I, typename P, int G, int VW, int U>
+__device__ __forceinline__ void spmm_chunk_role(const SpmmParams<V, I, P>& p, int64_t bid) {
+  using CA = typename MmChunkAcc<A>::T;
+  constexpr int kGroups = kWave / G;
+  const int lane = threadIdx.x % kWave;
+  const int64_t item = bid * kMmWaves + threadIdx.x / kWave;
+  const int64_t c = item / p.n_tiles, tile = item % p.n_tiles;
+  if (c >= p.n_chunks) return;
+  const int grp = lane / G, g = lane % G;
+  const int64_t n0 = tile * (G * VW) + (int64_t)g * VW;
+  const bool colok = n0 < p.n_dense;
+  const int64_t li = p.chunk_long[c];
+  const int64_t row = p.long_rows[li];
+  const int64_t c0 = p.long_chunk_ptr[li];
+  const int64_t nc = p.long_chunk_ptr[li + 1] - c0;
+  const int64_t rs = (int64_t)p.ptr[row], re = (int64_t)p.ptr[row + 1];
+  const int64_t s = rs + (c - c0) * p.chunk_nnz;
+  const int64_t e = s + p.chunk_nnz < re ? s + p.chunk_nnz : re;
+  CA acc[VW];
+#pragma unroll
+  for (int k = 0; k < VW; ++k) acc[k].init();
+  if (colok) mm_accumulate<VW, U>(acc, p.idx, p.val, p.B, p.ldb, n0, s + grp, e, kGroups);
+#pragma unroll
+  for (int k = 0; k < VW; ++k) cross_group_reduce<G>(acc[k]);
+  if (nc == 1) {
+    if (grp == 0 && colok)
+#pragma unroll
+      for (int k = 0; k < VW; ++k) write_out(p.C, row * p.ldc + n0 + k, acc[k].value(), p.alpha, p.beta);
+    return;
+  }
+  CA* parts = reinterpret_cast<CA*>(p.partials);
+  if (grp == 0 && colok)
+#pragma unroll
+    for (int k = 0; k < VW; ++k) store_partial_mm(parts + c * p.n_dense + n0 + k, acc[k]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  int last = 0;
+  if (lane == 0) {
+    const int old = __hip_atomic_fetch_add(p.counters + li * p.n_tiles + tile, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+    last = (old == (int)nc - 1);
+  }
+  last = __builtin_amdgcn_readfirstlane(last);
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  CA tot[VW];
+#pragma unroll
+  for (int k = 0; k < VW; ++k) tot[k].init();
+  if (colok)
+    for (int64_t q = grp; q < nc; q += kGroups)
+#pragma unroll
+      for (int k = 0; k < VW; ++k) tot[k].merge(load_partial_mm(parts + (c0 + q) * p.n_dense + n0 + k));
+#pragma unroll
+  for (int k = 0; k < VW; ++k) cross_group_reduce<G>(tot[k]);
+  if (grp == 0 && colok)
+#pragma unroll
+    for (int k = 0; k < VW; ++k) write_out(p.C, row * p.ldc + n0 + k, tot[k].value(), p.alpha, p.beta);
+  if (lane == 0)
+    __hip_atomic_store(p.counters + li * p.n_tiles + tile, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename V, typename A, typename I, typename P, int G, int VW, int U>
+__global__ __launch_bounds__(kMmBlock) void spmm_csr_kernel(SpmmParams<V, I, P> p) {
+  const int64_t bid = blockIdx.x;
+  if (bid < p.n_chunk_blocks) {
+    spmm_chunk_role<V, A, I, P, G, VW, U>(p, bid);
+    return;
+  }
+  int64_t sb = bid - p.n_chunk_blocks;
+  if (p.xcd_remap) sb = xcd_remap_block(sb, (int64_t)gridDim.x - p.n_chunk_blocks);
+  const int64_t tile = sb % p.n_tiles, rb = sb / p.n_tiles;
+  const int64_t row = rb * (kMmBlock / G) + threadIdx.x / G;
+  if (row >= p.n_rows) return;
+  const int g = threadIdx.x % G;
+  const int64_t n0 = tile * (G * VW) + (int64_t)g * VW;
+  if (n0 >= p.n_dense) return;  // lanes past the last column own nothing
+  const int64_t rs = (int64_t)p.ptr[row], re = (int64_t)p.ptr[row + 1];
+  if (re - rs > p.short_max) return;  // chunked elsewhere
+  A acc[VW];
+#pragma unroll
+  for (int k = 0; k < VW; ++k) acc[k].init();
+  mm_accumulate<VW, U>(acc, p.idx, p.val, p.B, p.ldb, n0, rs, re, 1);
+#pragma unroll
+  for (int k = 0; k < VW; ++k) write_out(p.C, row * p.ldc + n0 + k, acc[k].value(), p.alpha, p.beta);
+}
+
+template <typename V, typename I>
+constexpr bool kHotSpmm = (std::is_same<V, float>::value || std::is_same<V, double>::value) &&
+                          std::is_same<I, int32_t>::value;
+
+template <typename V, typename A, typename I, typename P, int G, int VW>
+int launch_spmm_gv(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t ldc,
+                   int64_t n_dense, double alpha, double beta, const fs_plan* pl, hipStream_t st) {
+  SpmmParams<V, I, P> p;
+  p.ptr = static_cast<const P*>(M->row_ptr);
+  p.idx = static_cast<const I*>(M->col_idx);
+  p.val = static_cast<const V*>(M->values);
+  p.B = static_cast<const V*>(B);
+  p.C = static_cast<V*>(C);
+  p.ldb = ldb;
+  p.ldc = ldc;
+  p.n_dense = n_dense;
+  p.n_rows = M->n_rows;
+  p.n_tiles = (n_dense + G * VW - 1) / (G * VW);
+  p.alpha = alpha;
+  p.beta = beta;
+  p.short_max = pl->short_max;
+  p.n_chunks = pl->n_chunks;
+  p.n_chunk_blocks = (pl->n_chunks * p.n_tiles + kMmWaves - 1) / kMmWaves;
+  p.chunk_nnz = pl->chunk_nnz;
+  p.long_rows = pl->long_rows;
+  p.long_chunk_ptr = pl->long_chunk_ptr;
+  p.chunk_long = pl->chunk_long;
+  p.counters = pl->counters;
+  p.partials = pl->partials;
+  p.xcd_remap = pl->xcd_remap;
+  const int64_t rows_per_block = kMmBlock / G;
+  const int64_t short_blocks = ((M->n_rows + rows_per_block - 1) / rows_per_block) * p.n_tiles;
+  const int64_t grid = p.n_chunk_blocks + short_blocks;
+  if (grid == 0) return FS_OK;
+  if (grid > 0x7fffffffLL)
+    return fail(FS_ERR_UNSUPPORTED, "fs_spmm_csr: grid of %lld blocks too large", (long long)grid);
+  spmm_csr_kernel<V, A, I, P, G, VW, 4><<<dim3((unsigned)grid), dim3(kMmBlock), 0, st>>>(p);
+  return check_launch("fs_spmm_csr");
+}
+
+template <typename V, typename A, typename I, typename P, int VW>
+int launch_spmm_g(int G, const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t ldc,
+                  int64_t n_dense, double alpha, double beta, const fs_plan* pl, hipStream_t st) {
+  if constexpr (kHotSpmm<V, I>) {
+    switch (G) {
+      case 1: return launch_spmm_gv<V, A, I, P, 1, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+      case 2: return launch_spmm_gv<V, A, I, P, 2, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+      case 4: return launch_spmm_gv<V, A, I, P, 4, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+      case 8: return launch_spmm_gv<V, A, I, P, 8, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+      case 16: return launch_spmm_gv<V, A, I, P, 16, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+      case 32: return launch_spmm_gv<V, A, I, P, 32, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+      default: return launch_spmm_gv<V, A, I, P, 64, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+    }
+  } else {
+    if (G <= 4) return launch_spmm_gv<V, A, I, P, 4, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+    if (G <= 16) return launch_spmm_gv<V, A, I, P, 16, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+    return launch_spmm_gv<V, A, I, P, 64, VW>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+  }
+}
+
+static inline int pow2_at_least(int64_t v) {
+  int g = 1;
+  while (g < v && g < 64) g <<= 1;
+  return g;
+}
+
+template <typename V, typename A, typename I, typename P>
+int launch_spmm(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t ldc, int64_t n_dense,
+                double alpha, double beta, const fs_plan* pl, int32_t col_tile, hipStream_t st) {
+  // Vector width: the widest of {4, 2, 1} (fp64: {2, 1}; others: 1) that
+  // divides n_dense and both leading dimensions and keeps B / C aligned.
+  int vw = 1;
+  if constexpr (kHotSpmm<V, I>) {
+    const int maxv = std::is_same<V, float>::value ? 4 : 2;
+    for (int v = maxv; v >= 2; v >>= 1) {
+      const uintptr_t al = (uintptr_t)(v * sizeof(V));
+      if (n_dense % v == 0 && ldb % v == 0 && ldc % v == 0 && ((uintptr_t)B % al) == 0 &&
+          ((uintptr_t)C % al) == 0) {
+        vw = v;
+        break;
+      }
+    }
+  }
+  int64_t tile = col_tile > 0 ? col_tile : n_dense;
+  if (tile % vw != 0) vw = 1;
+  int G = pow2_at_least((tile + vw - 1) / vw);
+  if (vw == 4) return launch_spmm_g<V, A, I, P, 4>(G, M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+  if (vw == 2) return launch_spmm_g<V, A, I, P, 2>(G, M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+  return launch_spmm_g<V, A, I, P, 1>(G, M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
+}
+
+template <typename V, typename A>
+int launch_spmm_idx(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t ldc, int64_t n_dense,
+                    double alpha, double beta, const fs_plan* pl, int32_t col_tile, hipStream_t st) {
+  if (M->index_type == FS_I32 && M->ptr_type == FS_I32) return launch_spmm<V, A, int32_t, int32_t>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, col_tile, st);
+  if (M->index_type == FS_I32 && M->ptr_type == FS_I64) return launch_spmm<V, A, int32_t, int64_t>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, col_tile, st);
+  if (M->index_type == FS_I64 && M->ptr_type == FS_I32) return launch_spmm<V, A, int64_t, int32_t>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, col_tile, st);
+  return launch_spmm<V, A, int64_t, int64_t>(M, B, ldb, C, ldc, n_dense, alpha, beta, pl, col_tile, st);
+}
+
+}  // namespace fs
+
+using namespace fs;
+
+extern "C" int fs_spmm_csr(const fs_csr* A, const void* B, int64_t ldb, void* C, int64_t ldc,
+                           int64_t n_dense, double alpha, double beta, const fs_plan* plan,
+                           int32_t col_tile, void* stream) {
+  int rc = check_csr(A, "fs_spmm_csr");
+  if (rc) return rc;
+  if (n_dense < 0) return fail(FS_ERR_VALUE, "fs_spmm_csr: n_dense must be >= 0");
+  if (col_tile < 0) return fail(FS_ERR_VALUE, "fs_spmm_csr: col_tile must be >= 0");
+  rc = check_plan(plan, A, n_dense, "fs_spmm_csr");
+  if (rc) return rc;
+  if (A->n_rows == 0 || n_dense == 0) return FS_OK;
+  if (!C) return fail(FS_ERR_VALUE, "fs_spmm_csr: C is null");
+  if (A->n_cols > 0 && !B) return fail(FS_ERR_VALUE, "fs_spmm_csr: B is null");
+  if (ldc < n_dense || (A->n_cols > 0 && ldb < n_dense))
+    return fail(FS_ERR_VALUE, "fs_spmm_csr: leading dimensions must be >= n_dense");
+  hipStream_t st = as_stream(stream);
+  const bool wide = plan->accumulate == FS_ACC_WIDE;
+  switch (A->value_type) {
+    case FS_F16: return launch_spmm_idx<f16_t, Plain<float>>(A, B, ldb, C, ldc, n_dense, alpha, beta, plan, col_tile, st);
+    case FS_BF16: return launch_spmm_idx<bf16_t, Plain<float>>(A, B, ldb, C, ldc, n_dense, alpha, beta, plan, col_tile, st);
+    case FS_F32:
+      return wide ? launch_spmm_idx<float, Plain<double>>(A, B, ldb, C, ldc, n_dense, alpha, beta, plan, col_tile, st)
+                  : launch_spmm_idx<float, Plain<float>>(A, B, ldb, C, ldc, n_dense, alpha, beta, plan, col_tile, st);
+    case FS_F64: return launch_spmm_idx<double, Plain<double>>(A, B, ldb, C, ldc, n_dense, alpha, beta, plan, col_tile, st);
+    case FS_C32: return launch_spmm_idx<c32_t, Plain<cplx<float>>>(A, B, ldb, C, ldc, n_dense, alpha, beta, plan, col_tile, st);
+    case FS_C64: return launch_spmm_idx<c64_t, Plain<cplx<float>>>(A, B, ldb, C, ldc, n_dense, alpha, beta, plan, col_tile, st);
+    case FS_C128: return launch_spmm_idx<c128_t, Plain<cplx<double>>>(A, B, ldb, C, ldc, n_dense, alpha, beta, plan, col_tile, st);
+    default: return fail(FS_ERR_TYPE, "fs_spmm_csr: unsupported value_type %d", A->value_type);
+  }
+}

@@ -1,0 +1,125 @@
+"""Row-split schedules and their device metadata (the ``fs_plan`` of the C ABI).
+
+A schedule sends every row of length <= ``short_max`` to the short-row role
+(groups of ``group_lanes`` lanes, natural row order) and cuts every longer row
+into chunks of ``chunk_nnz`` entries reduced by one wavefront each.  The plan
+holds, in device memory: the long-row ids, each long row's first chunk, the
+long-row ordinal of every chunk, the per-row arrival counters (zero between
+calls) and the chunk-partial workspace.
+
+This replaces the reference's row-length buckets (spmv_csr.py:128-153,
+345-391; spmm_csr.py:395-477; spmm_csr_opt_alg2.py:60-166, 307-335): the
+short role needs no row list at all, and the long-row split is computed on
+the device instead of the host loop of spmm_csr.py:417-441.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import torch
+
+from . import _lib
+
+
+@dataclasses.dataclass(frozen=True)
+class Schedule:
+    group_lanes: int = 8      # SpMV lanes per short row (1..64, power of two)
+    short_max: int = 128      # longest row of the short role
+    chunk_nnz: int = 512      # entries per long-row chunk (multiple of 64)
+    unroll: int = 4           # entries in flight per lane
+    xcd_remap: int = 0        # 1: contiguous row ranges per XCD
+    col_tile: int = 0         # SpMM dense columns per lane group (0: all)
+
+    def key(self) -> tuple:
+        return dataclasses.astuple(self)
+
+
+def _pow2_ceil(v: float) -> int:
+    p = 1
+    while p < v and p < 64:
+        p <<= 1
+    return p
+
+
+def default_spmv_schedule(n_rows: int, nnz: int) -> Schedule:
+    """Fixed rule used when no selector is supplied: group width near half the
+    mean row length, rows over 16 entries per lane chunked in 512-entry chunks."""
+    mean = nnz / n_rows if n_rows > 0 else 0.0
+    s = _pow2_ceil(max(mean / 2.0, 1.0))
+    return Schedule(group_lanes=s, short_max=max(16 * s, 64), chunk_nnz=512, unroll=4)
+
+
+def default_spmm_schedule(n_rows: int, nnz: int, n_dense: int) -> Schedule:
+    return Schedule(group_lanes=1, short_max=256, chunk_nnz=256, unroll=4, col_tile=0)
+
+
+class RowSplitPlan:
+    """Device metadata of one schedule for one matrix (see module docstring).
+
+    Calls that share a plan must be ordered on one stream: the arrival
+    counters and partials are reused by every call.
+    """
+
+    def __init__(self, indptr: torch.Tensor, n_rows: int, value_dtype, schedule: Schedule,
+                 accumulate: int, row_lengths: torch.Tensor | None = None):
+        device = indptr.device
+        self.schedule = schedule
+        self.accumulate = int(accumulate)
+        self.value_code = _lib.value_code(value_dtype)
+        self.device = device
+        if row_lengths is None:
+            row_lengths = (indptr[1:].to(torch.int64) - indptr[:-1].to(torch.int64)) if n_rows > 0 \
+                else torch.zeros(0, dtype=torch.int64, device=device)
+        long_mask = row_lengths > schedule.short_max
+        self.long_rows = torch.nonzero(long_mask, as_tuple=False).flatten().to(torch.int64)
+        self.n_long = int(self.long_rows.numel())
+        if self.n_long:
+            lens = row_lengths[self.long_rows]
+            per = (lens + schedule.chunk_nnz - 1) // schedule.chunk_nnz
+            self.long_chunk_ptr = torch.zeros(self.n_long + 1, dtype=torch.int64, device=device)
+            torch.cumsum(per, 0, out=self.long_chunk_ptr[1:])
+            self.n_chunks = int(self.long_chunk_ptr[-1].item())
+            self.chunk_long = torch.repeat_interleave(
+                torch.arange(self.n_long, dtype=torch.int32, device=device), per)
+        else:
+            self.long_chunk_ptr = torch.zeros(1, dtype=torch.int64, device=device)
+            self.n_chunks = 0
+            self.chunk_long = torch.zeros(0, dtype=torch.int32, device=device)
+        self._work = {}  # n_dense -> (counters, partials)
+        self._structs = {}
+
+    def struct(self, n_dense: int = 1) -> _lib.FsPlan:
+        """fs_plan for a call with n_dense dense columns (1 for SpMV)."""
+        key = max(int(n_dense), 1)
+        st = self._structs.get(key)
+        if st is not None:
+            return st
+        if self.n_long:
+            counters = torch.zeros(self.n_long * key, dtype=torch.int32, device=self.device)
+            nbytes = _lib.partials_bytes(self.value_code, self.accumulate, self.n_chunks, key)
+            partials = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=self.device)
+        else:
+            counters = torch.zeros(1, dtype=torch.int32, device=self.device)
+            partials = torch.empty(8, dtype=torch.uint8, device=self.device)
+            nbytes = 0
+        self._work[key] = (counters, partials)
+        sch = self.schedule
+        st = _lib.FsPlan(
+            accumulate=self.accumulate,
+            group_lanes=sch.group_lanes,
+            xcd_remap=sch.xcd_remap,
+            unroll=sch.unroll,
+            short_max=sch.short_max,
+            chunk_nnz=sch.chunk_nnz,
+            n_long=self.n_long,
+            long_rows=_lib.ptr(self.long_rows),
+            long_chunk_ptr=_lib.ptr(self.long_chunk_ptr),
+            n_chunks=self.n_chunks,
+            chunk_long=_lib.ptr(self.chunk_long),
+            counters=counters.data_ptr(),
+            partials=partials.data_ptr(),
+            partials_bytes=int(partials.numel()),
+        )
+        self._structs[key] = st
+        return st

@@ -1,0 +1,40 @@
+"""CSR SpMV / SpMM operators (the reference's sparse_operations names for this path)."""
+
+from ._common import SUPPORTED_INDEX_DTYPES, SUPPORTED_VALUE_DTYPES
+from .spmv_csr import (
+    PreparedCsrSpmv,
+    flagsparse_spmv_coo_tocsr,
+    flagsparse_spmv_csr,
+    prepare_spmv_coo_tocsr,
+    prepare_spmv_csr,
+)
+from .spmm_csr import (
+    PreparedCsrSpmmOpt,
+    benchmark_spmm_opt_case,
+    flagsparse_spmm_csr,
+    flagsparse_spmm_csr_opt,
+    prepare_spmm_csr_opt,
+)
+from .spmm_csr_opt_alg2 import (
+    PreparedCsrSpmmOptAlg2,
+    flagsparse_spmm_csr_opt_alg2,
+    prepare_spmm_csr_opt_alg2,
+)
+
+__all__ = [
+    "PreparedCsrSpmv",
+    "PreparedCsrSpmmOpt",
+    "PreparedCsrSpmmOptAlg2",
+    "SUPPORTED_INDEX_DTYPES",
+    "SUPPORTED_VALUE_DTYPES",
+    "benchmark_spmm_opt_case",
+    "flagsparse_spmm_csr",
+    "flagsparse_spmm_csr_opt",
+    "flagsparse_spmm_csr_opt_alg2",
+    "flagsparse_spmv_coo_tocsr",
+    "flagsparse_spmv_csr",
+    "prepare_spmm_csr_opt",
+    "prepare_spmm_csr_opt_alg2",
+    "prepare_spmv_coo_tocsr",
+    "prepare_spmv_csr",
+]

@@ -1,0 +1,592 @@
+"""CSR SpMM on gfx950 behind the reference's operator interface.
+
+Mirrors ``flagsparse/sparse_operations/spmm_csr.py``: ``flagsparse_spmm_csr``
+(ALG1 default path), ``prepare_spmm_csr_opt`` / ``flagsparse_spmm_csr_opt``,
+the error / tolerance helpers and the validation contract.  Kernels are the
+HIP ones of ``csrc/spmm.hip`` (C ABI ``fs_spmm_csr``).
+
+Numerics: the ALG1 default path accumulates fp32 in fp64 and rounds once
+(spmm_csr.py:544-551 casts data and B to fp64), fp16 / bf16 in fp32; the opt
+path accumulates fp32 / fp64 natively (chunked long rows of fp32 with
+compensated summation).  Extensions: trailing ``alpha`` / ``beta``.
+"""
+
+from __future__ import annotations
+
+import time
+
+import torch
+
+from .. import _lib
+from ..rowsplit import RowSplitPlan, Schedule, default_spmm_schedule
+from ._common import (
+    SUPPORTED_INDEX_DTYPES,
+    SUPPORTED_VALUE_DTYPES,
+    _INDEX_LIMIT_INT32,
+    _build_random_csr,
+    _build_random_dense,
+    _benchmark_cuda_op,
+    _is_complex_dtype,
+    _tolerance_for_dtype,
+)
+
+SUPPORTED_SPMM_VALUE_DTYPES = SUPPORTED_VALUE_DTYPES
+
+
+# ------------------------------------------------------------ error metrics
+
+def _spmm_relative_threshold(value_dtype):
+    """Max-relative-error threshold per dtype (spmm_csr.py:6-15)."""
+    if value_dtype == torch.float16:
+        return 5e-3
+    if value_dtype == torch.bfloat16:
+        return 1e-2
+    if value_dtype in (torch.float32, torch.complex64):
+        return 1e-6
+    if value_dtype in (torch.float64, torch.complex128):
+        return 1e-12
+    return 1e-6
+
+
+def _spmm_coo_reference_tolerance(value_dtype):
+    """(atol, rtol) of the opt reference check (spmm_csr.py:18-27)."""
+    if value_dtype == torch.float16:
+        return 2e-3, 2e-3
+    if value_dtype == torch.bfloat16:
+        return 1e-1, 1e-1
+    if value_dtype in (torch.float32, torch.complex64):
+        return 1e-4, 1e-2
+    if value_dtype in (torch.float64, torch.complex128):
+        return 1e-12, 1e-10
+    return 1e-6, 1e-5
+
+
+def _spmm_error_metrics(candidate, reference):
+    """max_abs, max_rel (= max_abs / max|ref|), sum_rel (spmm_csr.py:30-74);
+    complex values are compared by magnitude as in the reference."""
+    if candidate.shape != reference.shape:
+        raise ValueError("candidate and reference must have the same shape, got "
+                         f"{candidate.shape} vs {reference.shape}")
+    if candidate.numel() == 0:
+        return {"max_abs_error": 0.0, "max_relative_error": 0.0, "sum_relative_error": 0.0,
+                "reference_max_magnitude": 0.0, "reference_sum_magnitude": 0.0}
+    if _is_complex_dtype(reference.dtype):
+        ref_mag = torch.abs(reference)
+        diff = torch.abs(torch.abs(candidate) - ref_mag)
+    else:
+        ref_mag = torch.abs(reference)
+        diff = torch.abs(candidate - reference)
+    max_abs = float(torch.max(diff).item())
+    ref_max = float(torch.max(ref_mag).item())
+    sum_abs = float(torch.sum(diff).item())
+    ref_sum = float(torch.sum(ref_mag).item())
+
+    def ratio(num, den):
+        if den == 0.0:
+            return 0.0 if num == 0.0 else float("inf")
+        return num / den
+
+    return {"max_abs_error": max_abs, "max_relative_error": ratio(max_abs, ref_max),
+            "sum_relative_error": ratio(sum_abs, ref_sum), "reference_max_magnitude": ref_max,
+            "reference_sum_magnitude": ref_sum}
+
+
+def _spmm_validation_metrics(candidate, reference):
+    """Metrics plus threshold / allclose verdicts (spmm_csr.py:77-90)."""
+    m = _spmm_error_metrics(candidate, reference)
+    thr = _spmm_relative_threshold(reference.dtype)
+    atol, rtol = _tolerance_for_dtype(reference.dtype)
+    m.update({"relative_threshold": thr, "matches_threshold": m["max_relative_error"] <= thr,
+              "strict_allclose_match": torch.allclose(candidate, reference, atol=atol, rtol=rtol)})
+    return m
+
+
+# --------------------------------------------------------------- validation
+
+def _prepare_spmm_csr_matrix(data, indices, indptr, shape):
+    """Checks and messages of spmm_csr.py:197-255, device checks fused in one
+    fs_csr_analyze pass.  Column indices become int32 and indptr int64 as in
+    the reference (:247-248)."""
+    if len(shape) != 2:
+        raise ValueError("shape must be a 2-tuple: (n_rows, n_cols)")
+    if data.ndim != 1 or indices.ndim != 1 or indptr.ndim != 1:
+        raise ValueError("data, indices, and indptr must be 1D tensors")
+    n_rows, n_cols = int(shape[0]), int(shape[1])
+    if n_rows < 0 or n_cols < 0:
+        raise ValueError("shape dimensions must be non-negative")
+    if indptr.numel() != n_rows + 1:
+        raise ValueError(f"indptr length must be n_rows+1={n_rows + 1}, got {indptr.numel()}")
+    if data.numel() != indices.numel():
+        raise ValueError("data and indices must have the same length (nnz)")
+    if not all(t.is_cuda for t in (data, indices, indptr)):
+        raise ValueError("data, indices, and indptr must be CUDA tensors")
+    if not all(t.device == data.device for t in (indices, indptr)):
+        raise ValueError("data, indices, and indptr must be on the same CUDA device")
+    if data.dtype not in SUPPORTED_SPMM_VALUE_DTYPES:
+        raise TypeError("data dtype must be one of: float16, bfloat16, float32, float64, "
+                        "complex64, complex128")
+    if indices.dtype not in SUPPORTED_INDEX_DTYPES:
+        raise TypeError("indices dtype must be torch.int32 or torch.int64")
+    if indptr.dtype not in SUPPORTED_INDEX_DTYPES:
+        raise TypeError("indptr dtype must be torch.int32 or torch.int64")
+    nnz = data.numel()
+    data = data.contiguous()
+    indices = indices.contiguous()
+    indptr = indptr.contiguous()
+    stats = _lib.analyze(_lib.make_csr(data, indices, indptr, n_rows, n_cols), data.device)
+    if stats[0] != 0:
+        raise ValueError("indptr[0] must be 0")
+    if stats[1] != nnz:
+        raise ValueError(f"indptr[-1] must equal nnz={nnz}, got {int(stats[1])}")
+    if stats[2] > 0:
+        raise ValueError("indptr must be nondecreasing")
+    if nnz > 0:
+        if stats[3] < 0 or stats[4] >= n_cols:
+            raise IndexError("indices out of range for n_cols")
+        if stats[4] > _INDEX_LIMIT_INT32:
+            raise ValueError("column indices exceed the int32 range supported by the Triton kernel")
+    kernel_indices = indices.to(torch.int32) if indices.dtype == torch.int64 else indices
+    kernel_indptr = indptr.to(torch.int64)
+    row_lengths = (kernel_indptr[1:] - kernel_indptr[:-1]) if n_rows > 0 \
+        else kernel_indptr.new_empty((0,))
+    max_row_nnz = int(stats[5]) if n_rows > 0 else 0
+    return data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths, max_row_nnz
+
+
+def _prepare_spmm_csr_inputs(data, indices, indptr, B, shape):
+    """Matrix checks plus B checks (spmm_csr.py:258-281)."""
+    if B.ndim != 2:
+        raise ValueError("B must be a 2D dense tensor")
+    (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
+     max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape)
+    if B.shape[0] != n_cols:
+        raise ValueError(f"B.shape[0] must be n_cols={n_cols}, got {B.shape[0]}")
+    if not B.is_cuda:
+        raise ValueError("B must be a CUDA tensor")
+    if B.device != data.device:
+        raise ValueError("B must be on the same CUDA device as the sparse matrix")
+    if B.dtype != data.dtype:
+        raise TypeError("B dtype must match data dtype")
+    return (data, kernel_indices, kernel_indptr, B.contiguous(), n_rows, n_cols,
+            int(B.shape[1]), row_lengths, max_row_nnz)
+
+
+def _select_spmm_alg1_warp_and_factor(n_dense_cols):
+    """ALG1 (warp, factor) table of spmm_csr.py:284-296 (recorded, and used to
+    resolve block_n defaults)."""
+    if n_dense_cols > 64:
+        return 32, 4
+    if n_dense_cols > 32:
+        return 32, 2
+    if n_dense_cols > 16:
+        return 32, 1
+    if n_dense_cols > 8:
+        return 16, 1
+    if n_dense_cols > 4:
+        return 8, 1
+    return 4, 1
+
+
+def _resolve_spmm_alg1_launch_config(n_dense_cols, max_row_nnz, block_n=None, block_nnz=None,
+                                     max_segments=None):
+    """block_n / block_nnz defaults and the explicit max_segments check
+    (spmm_csr.py:299-334)."""
+    warp_size, factor = _select_spmm_alg1_warp_and_factor(n_dense_cols)
+    if block_n is None:
+        block_n = warp_size * factor
+    if block_nnz is None:
+        block_nnz = warp_size
+    if block_n <= 0 or block_nnz <= 0:
+        raise ValueError("block_n and block_nnz must be positive when provided")
+    if max_segments is not None and max_segments <= 0:
+        raise ValueError("max_segments must be positive when provided")
+    required = (max_row_nnz + block_nnz - 1) // block_nnz if max_row_nnz > 0 else 0
+    if max_segments is not None and required > int(max_segments):
+        raise ValueError(
+            "row nnz requires more CSR segments than the explicit max_segments override allows: "
+            f"required {required}, provided {int(max_segments)}")
+    return {"block_n": int(block_n), "block_nnz": int(block_nnz),
+            "max_segments": None if max_segments is None else int(max_segments),
+            "required_segments": int(required), "warp_size": int(warp_size),
+            "factor": int(factor), "max_row_nnz": int(max_row_nnz),
+            "auto_max_segments": max_segments is None}
+
+
+# ---------------------------------------------------------------- launches
+
+def _plan_for(cache, kernel_indptr, n_rows, dtype, accumulate, schedule, row_lengths):
+    key = (int(accumulate), schedule.key())
+    p = cache.get(key)
+    if p is None:
+        p = RowSplitPlan(kernel_indptr, n_rows, dtype, schedule, accumulate,
+                         row_lengths=row_lengths)
+        cache[key] = p
+    return p
+
+
+def _launch_spmm(csr, plan, B, n_rows, n_dense, dtype, device, out=None, alpha=1.0, beta=0.0,
+                 col_tile=0):
+    C = out if out is not None else torch.empty((n_rows, n_dense), dtype=dtype, device=device)
+    if n_rows == 0 or n_dense == 0:
+        return C
+    _lib.spmm(csr, B, B.stride(0) if B.numel() else n_dense, C, C.stride(0), n_dense, alpha, beta,
+              plan.struct(n_dense), col_tile, device)
+    return C
+
+
+def _hip_spmm_csr_impl(data, indices, indptr, B, n_rows, n_dense_cols, block_n, block_nnz,
+                       schedule=None, row_lengths=None, n_cols=None, out=None, alpha=1.0,
+                       beta=0.0):
+    """ALG1 default path (spmm_csr.py:525-599): fp32 widened to fp64 in the
+    kernel, fp16 / bf16 in fp32, fp64 / complex natively."""
+    device, dtype = data.device, data.dtype
+    if n_rows == 0 or n_dense_cols == 0 or B.shape[0] == 0:
+        if out is None:
+            return torch.zeros((n_rows, n_dense_cols), dtype=dtype, device=device)
+        return out.mul_(beta) if beta != 0.0 else out.zero_()
+    n_cols = int(B.shape[0]) if n_cols is None else n_cols
+    acc = _lib.FS_ACC_WIDE if dtype == torch.float32 else _lib.FS_ACC_NATIVE
+    sch = schedule or default_spmm_schedule(n_rows, data.numel(), n_dense_cols)
+    plan = RowSplitPlan(indptr, n_rows, dtype, sch, acc, row_lengths=row_lengths)
+    csr = _lib.make_csr(data, indices, indptr, n_rows, n_cols)
+    return _launch_spmm(csr, plan, B, n_rows, n_dense_cols, dtype, device, out, alpha, beta,
+                        sch.col_tile)
+
+
+_triton_spmm_csr_impl = _hip_spmm_csr_impl
+
+
+def flagsparse_spmm_csr(data, indices, indptr, B, shape, block_n=None, block_nnz=None,
+                        max_segments=None, out=None, return_time=False, alpha=1.0, beta=0.0):
+    """C = A @ B (spmm_csr.py:2088-2157): row-major, non-transposed B and C.
+
+    ``block_n`` / ``block_nnz`` / ``max_segments`` keep the reference's checks;
+    they do not change the HIP schedule.  Unlike the reference, no device
+    synchronisation happens unless ``return_time`` is set.
+    """
+    if block_n is not None and block_n <= 0:
+        raise ValueError("block_n must be positive when provided")
+    if block_nnz is not None and block_nnz <= 0:
+        raise ValueError("block_nnz must be positive when provided")
+    if max_segments is not None and max_segments <= 0:
+        raise ValueError("max_segments must be positive when provided")
+    (data, kernel_indices, kernel_indptr, B, n_rows, n_cols, n_dense_cols, row_lengths,
+     max_row_nnz) = _prepare_spmm_csr_inputs(data, indices, indptr, B, shape)
+    _resolve_spmm_alg1_launch_config(n_dense_cols, max_row_nnz, block_n=block_n,
+                                     block_nnz=block_nnz, max_segments=max_segments)
+    if out is not None:
+        if not out.is_cuda:
+            raise ValueError("out must be a CUDA tensor")
+        if out.device != data.device:
+            raise ValueError("out must be on the same CUDA device as the inputs")
+        if tuple(out.shape) != (n_rows, n_dense_cols) or out.dtype != data.dtype:
+            raise ValueError("out shape/dtype must match result")
+    if float(beta) != 0.0 and out is None:
+        raise ValueError("beta != 0 requires out (the C that is scaled and accumulated)")
+    direct = out is not None and out.is_contiguous()
+    t0 = None
+    if return_time:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+    C = _hip_spmm_csr_impl(data, kernel_indices, kernel_indptr, B, n_rows, n_dense_cols,
+                           block_n=block_n, block_nnz=block_nnz, row_lengths=row_lengths,
+                           n_cols=n_cols, out=out if direct else None, alpha=alpha, beta=beta)
+    elapsed_ms = None
+    if return_time:
+        torch.cuda.synchronize()
+        elapsed_ms = (time.perf_counter() - t0) * 1000.0
+    if out is not None and C is not out:
+        out.copy_(C)
+        C = out
+    if return_time:
+        return C, elapsed_ms
+    return C
+
+
+# --------------------------------------------------------------------- opt
+
+class PreparedCsrSpmmOpt:
+    """Cached metadata for repeated SpMM-opt calls (spmm_csr.py:337-392),
+    including the reference's bucket lists and 256-entry split parts (for
+    introspection; the HIP kernel runs the row-split plan)."""
+
+    __slots__ = (
+        "data", "kernel_indices", "kernel_indptr", "shape", "n_rows", "n_cols", "row_lengths",
+        "max_row_nnz", "row_buckets", "supports_opt", "long_part_rows", "long_part_starts",
+        "long_part_ends", "long_row_ids", "long_row_part_ptr", "long_row_fallback_only",
+        "schedule", "_plans", "_csr",
+    )
+
+    def __init__(self, data, kernel_indices, kernel_indptr, shape, n_rows, n_cols, row_lengths,
+                 max_row_nnz, row_buckets, long_part_rows, long_part_starts, long_part_ends,
+                 long_row_ids, long_row_part_ptr, long_row_fallback_only, schedule=None):
+        self.data = data
+        self.kernel_indices = kernel_indices
+        self.kernel_indptr = kernel_indptr
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.n_rows = int(n_rows)
+        self.n_cols = int(n_cols)
+        self.row_lengths = row_lengths
+        self.max_row_nnz = int(max_row_nnz)
+        self.row_buckets = row_buckets
+        self.supports_opt = data.dtype in (torch.float32, torch.float64)
+        self.long_part_rows = long_part_rows
+        self.long_part_starts = long_part_starts
+        self.long_part_ends = long_part_ends
+        self.long_row_ids = long_row_ids
+        self.long_row_part_ptr = long_row_part_ptr
+        self.long_row_fallback_only = bool(long_row_fallback_only)
+        self.schedule = schedule
+        self._plans = {}
+        self._csr = None
+
+    def csr_struct(self):
+        if self._csr is None:
+            self._csr = _lib.make_csr(self.data, self.kernel_indices, self.kernel_indptr,
+                                      self.n_rows, self.n_cols)
+        return self._csr
+
+    def plan(self, accumulate, n_dense, schedule=None):
+        sch = schedule or self.schedule or default_spmm_schedule(self.n_rows, self.data.numel(),
+                                                                 n_dense)
+        return _plan_for(self._plans, self.kernel_indptr, self.n_rows, self.data.dtype,
+                         accumulate, sch, self.row_lengths)
+
+
+_SPMM_OPT_BUCKET_SPECS = (
+    {"max_row_nnz": 32, "kind": "batched", "batch_rows": 8, "block_nnz": 32},
+    {"max_row_nnz": 128, "kind": "batched", "batch_rows": 4, "block_nnz": 64},
+    {"max_row_nnz": 512, "kind": "vector", "batch_rows": 1, "block_nnz": 128},
+    {"max_row_nnz": 2048, "kind": "vector", "batch_rows": 1, "block_nnz": 128},
+    {"max_row_nnz": None, "kind": "split", "batch_rows": 1, "block_nnz": 256},
+)
+_SPMM_OPT_LONG_ROW_THRESHOLD = 2048
+
+
+def _select_spmm_opt_block_n(n_dense_cols):
+    """Column tile of the opt path (spmm_csr.py:405-414)."""
+    for cap in (8, 16, 32, 64):
+        if n_dense_cols <= cap:
+            return cap
+    return 128
+
+
+def _build_spmm_opt_split_metadata(kernel_indptr, long_rows, part_block_nnz):
+    """Parts [s, min(s + part, e)) of each long row (spmm_csr.py:417-441),
+    computed on the device: (part_rows, part_starts, part_ends, row_part_ptr)."""
+    device = kernel_indptr.device
+    rows64 = long_rows.to(torch.int64)
+    starts = kernel_indptr[rows64].to(torch.int64)
+    ends = kernel_indptr[rows64 + 1].to(torch.int64)
+    per = (ends - starts + part_block_nnz - 1) // part_block_nnz
+    row_part_ptr = torch.zeros(rows64.numel() + 1, dtype=torch.int64, device=device)
+    if rows64.numel():
+        torch.cumsum(per, 0, out=row_part_ptr[1:])
+    owner = torch.repeat_interleave(torch.arange(rows64.numel(), device=device), per)
+    k = torch.arange(owner.numel(), device=device, dtype=torch.int64) - row_part_ptr[owner]
+    p_start = starts[owner] + k * part_block_nnz
+    p_end = torch.minimum(p_start + part_block_nnz, ends[owner])
+    row_dtype = long_rows.dtype if long_rows.numel() > 0 else torch.int32
+    return rows64[owner].to(row_dtype), p_start, p_end, row_part_ptr
+
+
+def _build_spmm_opt_buckets(row_lengths):
+    """Row lists of the five opt buckets (spmm_csr.py:444-477)."""
+    device = row_lengths.device
+    row_index_dtype = torch.int32 if int(row_lengths.numel()) - 1 <= _INDEX_LIMIT_INT32 else torch.int64
+    buckets = []
+    lower = 0
+    long_rows = torch.empty((0,), dtype=row_index_dtype, device=device)
+    for spec in _SPMM_OPT_BUCKET_SPECS:
+        upper = spec["max_row_nnz"]
+        if upper is None:
+            mask = row_lengths > lower
+        elif lower == 0:
+            mask = row_lengths <= upper
+        else:
+            mask = (row_lengths > lower) & (row_lengths <= upper)
+        rows = torch.nonzero(mask, as_tuple=False).flatten().to(row_index_dtype)
+        if rows.numel() > 0:
+            buckets.append({"kind": spec["kind"], "rows": rows, "batch_rows": spec["batch_rows"],
+                            "block_nnz": spec["block_nnz"]})
+            if spec["kind"] == "split":
+                long_rows = rows
+        if upper is not None:
+            lower = upper
+    return buckets, long_rows
+
+
+def prepare_spmm_csr_opt(data, indices, indptr, shape, schedule=None):
+    """Validate once and cache metadata (spmm_csr.py:480-522)."""
+    (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
+     max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape)
+    row_buckets, long_rows = _build_spmm_opt_buckets(row_lengths)
+    part_rows, part_starts, part_ends, part_ptr = _build_spmm_opt_split_metadata(
+        kernel_indptr, long_rows, 256)
+    return PreparedCsrSpmmOpt(
+        data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
+        n_rows=n_rows, n_cols=n_cols, row_lengths=row_lengths, max_row_nnz=max_row_nnz,
+        row_buckets=row_buckets, long_part_rows=part_rows, long_part_starts=part_starts,
+        long_part_ends=part_ends, long_row_ids=long_rows, long_row_part_ptr=part_ptr,
+        long_row_fallback_only=False, schedule=schedule)
+
+
+def _check_opt_B(prepared, B):
+    if not prepared.supports_opt:
+        raise TypeError("spmm opt only supports float32 and float64")
+    if not B.is_cuda:
+        raise ValueError("B must be a CUDA tensor")
+    if B.device != prepared.data.device:
+        raise ValueError("B must be on the same CUDA device as the sparse matrix")
+    if B.dtype != prepared.data.dtype:
+        raise TypeError("B dtype must match sparse matrix dtype")
+    if B.shape[0] != prepared.n_cols:
+        raise ValueError(f"B.shape[0] must be n_cols={prepared.n_cols}, got {B.shape[0]}")
+    if B.ndim != 2:
+        raise ValueError("B must be a 2D dense tensor")
+
+
+def _hip_spmm_csr_impl_opt_prepared(prepared, B, out=None, alpha=1.0, beta=0.0, schedule=None):
+    """Opt path (spmm_csr.py:2034-2056): native accumulation; returns (C, False)."""
+    _check_opt_B(prepared, B)
+    B = B.contiguous()
+    n_dense = int(B.shape[1])
+    plan = prepared.plan(_lib.FS_ACC_NATIVE, n_dense, schedule)
+    C = _launch_spmm(prepared.csr_struct(), plan, B, prepared.n_rows, n_dense, B.dtype, B.device,
+                     out, alpha, beta, plan.schedule.col_tile)
+    return C, False
+
+
+_triton_spmm_csr_impl_opt_prepared = _hip_spmm_csr_impl_opt_prepared
+
+
+def flagsparse_spmm_csr_opt(data=None, indices=None, indptr=None, B=None, shape=None,
+                            prepared=None, out=None, return_time=False, alpha=1.0, beta=0.0):
+    """Native fp32 / fp64 SpMM-opt (spmm_csr.py:2160-2214)."""
+    if prepared is not None and not isinstance(prepared, PreparedCsrSpmmOpt):
+        raise TypeError("prepared must be a PreparedCsrSpmmOpt instance")
+    if prepared is None:
+        if any(a is None for a in (data, indices, indptr, shape)):
+            raise ValueError(
+                "data, indices, indptr, and shape are required when prepared is not provided")
+        prepared = prepare_spmm_csr_opt(data, indices, indptr, shape)
+    elif shape is not None:
+        sh = (int(shape[0]), int(shape[1]))
+        if sh != prepared.shape:
+            raise ValueError(f"shape {sh} does not match prepared.shape {prepared.shape}")
+    if B is None:
+        raise ValueError("B is required")
+    if not prepared.supports_opt:
+        raise TypeError("flagsparse_spmm_csr_opt only supports float32 and float64")
+    if not B.is_cuda:
+        raise ValueError("B must be a CUDA tensor")
+    if B.device != prepared.data.device:
+        raise ValueError("B must be on the same CUDA device as sparse matrix data")
+    if out is not None:
+        if not out.is_cuda:
+            raise ValueError("out must be a CUDA tensor")
+        if out.device != prepared.data.device:
+            raise ValueError("out must be on the same CUDA device as sparse matrix data")
+        if tuple(out.shape) != (prepared.n_rows, int(B.shape[1])) or out.dtype != prepared.data.dtype:
+            raise ValueError("out shape/dtype must match result")
+    if float(beta) != 0.0 and out is None:
+        raise ValueError("beta != 0 requires out (the C that is scaled and accumulated)")
+    direct = out is not None and out.is_contiguous()
+    t0 = None
+    if return_time:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+    C, _ = _triton_spmm_csr_impl_opt_prepared(prepared, B, out=out if direct else None,
+                                              alpha=alpha, beta=beta)
+    elapsed_ms = None
+    if return_time:
+        torch.cuda.synchronize()
+        elapsed_ms = (time.perf_counter() - t0) * 1000.0
+    if out is not None and C is not out:
+        out.copy_(C)
+        C = out
+    if return_time:
+        return C, elapsed_ms
+    return C
+
+
+def _spmm_opt_reference_error(candidate, reference, value_dtype):
+    """max(|d| / (atol + rtol |ref|)) with the opt tolerances (spmm_csr.py:2217-2223)."""
+    atol, rtol = _spmm_coo_reference_tolerance(value_dtype)
+    if candidate.numel() == 0:
+        return 0.0
+    return float(torch.max(torch.abs(candidate - reference) /
+                           (atol + rtol * torch.abs(reference))).item())
+
+
+def _torch_reference_spmm(data, indices, indptr, B, shape, value_dtype):
+    """torch.sparse.mm in the promoted dtype (fp32 -> fp64), as the reference's
+    benchmarks do (spmm_csr.py:2265-2278)."""
+    ref_dtype = torch.float64 if value_dtype == torch.float32 else value_dtype
+    csr = torch.sparse_csr_tensor(indptr.to(torch.int64), indices.to(torch.int64),
+                                  data.to(ref_dtype), size=shape, device=data.device)
+    return torch.sparse.mm(csr, B.to(ref_dtype)).to(value_dtype)
+
+
+def benchmark_spmm_opt_case(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32,
+                            value_dtype=torch.float32, index_dtype=torch.int32, warmup=20,
+                            iters=200, run_cusparse=True):
+    """Base vs opt timing and error against the promoted torch reference
+    (spmm_csr.py:2226-2348).  ``cu_ms`` is None: there is no cuSPARSE."""
+    if value_dtype not in (torch.float32, torch.float64):
+        raise TypeError("benchmark_spmm_opt_case only supports float32 and float64")
+    device = torch.device("cuda")
+    data, indices, indptr = _build_random_csr(n_rows, n_cols, nnz, value_dtype, index_dtype, device)
+    B = _build_random_dense((n_cols, n_dense_cols), value_dtype, device)
+    shape = (n_rows, n_cols)
+    prepared = prepare_spmm_csr_opt(data, indices, indptr, shape)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    base = flagsparse_spmm_csr(data, indices, indptr, B, shape)
+    torch.cuda.synchronize()
+    base_first = (time.perf_counter() - t0) * 1000.0
+    base, base_ms = _benchmark_cuda_op(lambda: flagsparse_spmm_csr(data, indices, indptr, B, shape),
+                                       warmup, iters)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    opt = flagsparse_spmm_csr_opt(B=B, prepared=prepared)
+    torch.cuda.synchronize()
+    opt_first = (time.perf_counter() - t0) * 1000.0
+    opt, opt_ms = _benchmark_cuda_op(lambda: flagsparse_spmm_csr_opt(B=B, prepared=prepared),
+                                     warmup, iters)
+    ref = _torch_reference_spmm(data, indices, indptr, B, shape, value_dtype)
+    pt_ms = None
+    try:
+        pt = torch.sparse_csr_tensor(indptr.to(torch.int64), indices.to(torch.int64), data,
+                                     size=shape, device=device)
+        _, pt_ms = _benchmark_cuda_op(lambda: torch.sparse.mm(pt, B), warmup, iters)
+    except Exception:
+        pt_ms = None
+    err_base = _spmm_opt_reference_error(base, ref, value_dtype)
+    err_opt = _spmm_opt_reference_error(opt, ref, value_dtype)
+    return {
+        "parameters": {"n_rows": n_rows, "n_cols": n_cols, "nnz": nnz,
+                       "n_dense_cols": n_dense_cols, "value_dtype": str(value_dtype),
+                       "index_dtype": str(index_dtype)},
+        "performance": {"base_ms": base_ms, "base_first_call_ms": base_first, "opt_ms": opt_ms,
+                        "opt_first_call_ms": opt_first, "pt_ms": pt_ms, "cu_ms": None,
+                        "opt_vs_base": base_ms / opt_ms if opt_ms > 0 else None,
+                        "opt_vs_pt": pt_ms / opt_ms if pt_ms is not None and opt_ms > 0 else None,
+                        "opt_vs_cu": None},
+        "verification": {"err_base": err_base, "err_opt": err_opt, "base_ok": err_base <= 1.0,
+                         "opt_ok": err_opt <= 1.0, "status": "PASS" if err_opt <= 1.0 else "FAIL"},
+        "backend_status": {"long_row_fallback_used": bool(prepared.long_row_fallback_only),
+                           "flagsparse_internal_route": "hip-row-split"},
+        "samples": {"base": base, "opt": opt, "reference": ref},
+    }
+
+
+__all__ = [
+    "SUPPORTED_SPMM_VALUE_DTYPES",
+    "PreparedCsrSpmmOpt",
+    "flagsparse_spmm_csr",
+    "flagsparse_spmm_csr_opt",
+    "prepare_spmm_csr_opt",
+    "benchmark_spmm_opt_case",
+    "Schedule",
+]

@@ -1,0 +1,162 @@
+/*
+ * flagsparse_hip.h -- C ABI of the gfx950 CSR SpMV / SpMM library
+ * (libflagsparse_hip.so).
+ *
+ * Plain pointers and sizes only: every array argument is a device pointer
+ * (HIP device memory), every call is enqueued on the caller's HIP stream
+ * (`stream` is a hipStream_t passed as void*; NULL means the legacy default
+ * stream), and calls are asynchronous on the device.  No entry point
+ * allocates device memory except where stated: the caller owns inputs,
+ * outputs, plans and workspaces.
+ *
+ * Every entry point returns FS_OK (0) or an FS_ERR_* status; fs_last_error()
+ * returns the message of the calling thread's last failure.
+ *
+ * Each entry point replaces one kernel-launch routine of the reference
+ * (paths relative to the reference repository, src/flagsparse/sparse_operations/):
+ *   fs_spmv_csr        spmv_csr.py:664-708   _triton_spmv_csr_impl_prepared
+ *                      spmv_csr.py:393-440   _triton_spmv_csr_impl_opt_prepared
+ *   fs_spmm_csr        spmm_csr.py:525-599   _triton_spmm_csr_impl
+ *                      spmm_csr.py:2034-2056 _triton_spmm_csr_impl_opt_prepared
+ *                      spmm_csr_opt_alg2.py:771-798 _triton_spmm_csr_impl_opt_alg2_prepared
+ *   fs_csr_analyze     spmv_csr.py:497-551 / spmm_csr.py:197-255, the
+ *                      host-synchronising validation reductions
+ *                      (indptr[0], indptr[-1], monotonicity, index range,
+ *                      max row length), fused into one device pass.
+ */
+#ifndef FLAGSPARSE_HIP_H
+#define FLAGSPARSE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- status codes */
+enum {
+  FS_OK = 0,
+  FS_ERR_VALUE = 1,       /* bad argument value or shape   -> ValueError  */
+  FS_ERR_TYPE = 2,        /* unsupported dtype combination -> TypeError   */
+  FS_ERR_INDEX = 3,       /* index out of range            -> IndexError  */
+  FS_ERR_HIP = 4,         /* HIP runtime failure           -> RuntimeError */
+  FS_ERR_UNSUPPORTED = 5  /* valid request this build cannot run          */
+};
+
+/* ------------------------------------------------------------ value types */
+enum {
+  FS_F16 = 0,
+  FS_BF16 = 1,
+  FS_F32 = 2,
+  FS_F64 = 3,
+  FS_C32 = 4,   /* complex32: (fp16 re, fp16 im) */
+  FS_C64 = 5,   /* complex64: (fp32 re, fp32 im) */
+  FS_C128 = 6   /* complex128 */
+};
+
+/* ------------------------------------------------------------ index types */
+enum { FS_I32 = 0, FS_I64 = 1 };
+
+/* ----------------------------------------------------- accumulation modes */
+enum {
+  FS_ACC_NATIVE = 0,  /* accumulate in the compute type of the values
+                         (fp16/bf16 -> fp32); chunked long rows of fp32 use
+                         compensated (TwoSum) fp32 summation                  */
+  FS_ACC_WIDE = 1     /* fp32 values accumulated in fp64 and rounded once
+                         (the reference's default path, spmv_csr.py:116-122) */
+};
+
+/* A CSR matrix resident in device memory. */
+typedef struct fs_csr {
+  int32_t value_type;  /* FS_F16 .. FS_C128                       */
+  int32_t index_type;  /* FS_I32 / FS_I64: column indices         */
+  int32_t ptr_type;    /* FS_I32 / FS_I64: row pointer            */
+  int32_t reserved;
+  int64_t n_rows;
+  int64_t n_cols;
+  int64_t nnz;
+  const void *row_ptr; /* n_rows + 1 entries                      */
+  const void *col_idx; /* nnz entries                             */
+  const void *values;  /* nnz entries of value_type               */
+} fs_csr;
+
+/*
+ * Row-split schedule shared by SpMV and SpMM.  Rows whose length is at most
+ * short_max are processed in natural order by groups of group_lanes lanes
+ * (SpMV) or by one lane group per row (SpMM).  Longer rows are listed in
+ * long_rows and cut into chunks of chunk_nnz entries; one wavefront handles a
+ * chunk and the last chunk of a row to finish sums the row's partials in
+ * chunk order (deterministic).  counters[n_long] must be zero before the
+ * first call; every call leaves them zero again.  partials holds one
+ * accumulator per chunk (SpMV) or one accumulator row of n_dense per chunk
+ * (SpMM); size it with fs_partials_bytes.
+ */
+typedef struct fs_plan {
+  int32_t accumulate;      /* FS_ACC_*                                     */
+  int32_t group_lanes;     /* SpMV lanes per short row: 1,2,4,8,16,32,64   */
+  int32_t xcd_remap;       /* 1: give each XCD a contiguous range of rows  */
+  int32_t unroll;          /* entries in flight per lane: 1, 2, 4 or 8     */
+  int64_t short_max;       /* longest row handled by the short-row role    */
+  int64_t chunk_nnz;       /* entries per long-row chunk (multiple of 64)  */
+  int64_t n_long;          /* number of listed long rows                   */
+  const int64_t *long_rows;      /* [n_long] row ids, ascending            */
+  const int64_t *long_chunk_ptr; /* [n_long + 1] first chunk of each row   */
+  int64_t n_chunks;
+  const int32_t *chunk_long;     /* [n_chunks] long-row ordinal of a chunk */
+  int32_t *counters;             /* [n_long] arrival counters              */
+  void *partials;                /* chunk partial sums                     */
+  int64_t partials_bytes;
+} fs_plan;
+
+/* Bytes of `partials` needed for n_chunks chunks (n_dense = 1 for SpMV). */
+int64_t fs_partials_bytes(int32_t value_type, int32_t accumulate, int64_t n_chunks,
+                          int64_t n_dense);
+
+/*
+ * y = alpha * A x + beta * y.  y is not read when beta == 0.  x has n_cols
+ * entries and y n_rows entries, both of A->value_type.
+ */
+int fs_spmv_csr(const fs_csr *A, const void *x, void *y, double alpha, double beta,
+                const fs_plan *plan, void *stream);
+
+/*
+ * C = alpha * A B + beta * C with B (n_cols x n_dense, leading dimension ldb)
+ * and C (n_rows x n_dense, leading dimension ldc), both row-major.  C is not
+ * read when beta == 0.  col_tile is the number of dense columns one lane
+ * group covers per pass (0 = automatic).
+ */
+int fs_spmm_csr(const fs_csr *A, const void *B, int64_t ldb, void *C, int64_t ldc,
+                int64_t n_dense, double alpha, double beta, const fs_plan *plan,
+                int32_t col_tile, void *stream);
+
+/*
+ * One device pass over row_ptr and col_idx.  Writes FS_STATS_LEN int64 values
+ * to stats (device memory):
+ *   [0] row_ptr[0]            [1] row_ptr[n_rows]
+ *   [2] #i with row_ptr[i+1] < row_ptr[i]
+ *   [3] min column index (INT64_MAX if nnz == 0)
+ *   [4] max column index (-1 if nnz == 0)
+ *   [5] max row length        [6] min row length (0 if n_rows == 0)
+ *   [7] sum of squared row lengths
+ *   [8] empty rows
+ *   [9..14] rows with length in (0,4], (4,16], (16,64], (64,256],
+ *           (256,4096], (4096,inf)
+ *   [15] entries in rows longer than 256
+ * Column statistics are taken over the first nnz entries of col_idx; the row
+ * statistics over the n_rows + 1 row pointers.
+ */
+#define FS_STATS_LEN 16
+int fs_csr_analyze(const fs_csr *A, int64_t *stats, void *stream);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *fs_last_error(void);
+
+/* Library version string. */
+const char *fs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLAGSPARSE_HIP_H */

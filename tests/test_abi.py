@@ -1,0 +1,149 @@
+"""The C ABI library (CPU only): it loads, exports every function the header
+declares, and rejects malformed calls with the documented status codes
+before anything is launched."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from flagsparse_public_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include", "flagsparse_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fs_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert names, "no declarations found"
+    for name in names:
+        assert hasattr(lib, name), f"{name} declared in the header but not exported"
+    # and the Python binding covers every declaration
+    assert set(names) <= set(_lib.exported_symbols())
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of fs_csr / fs_plan have the C compiler's offsets."""
+    import subprocess
+    fields = {"fs_csr": [f for f, _ in _lib.FsCsr._fields_],
+              "fs_plan": [f for f, _ in _lib.FsPlan._fields_]}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){"]
+    for st, fs in fields.items():
+        lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for f in fs:
+            lines.append(f'printf("{st}.{f} %zu\\n", offsetof({st}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                 check=True).stdout.splitlines())
+    for st, cls in (("fs_csr", _lib.FsCsr), ("fs_plan", _lib.FsPlan)):
+        assert int(out[st]) == ctypes.sizeof(cls)
+        for f in fields[st]:
+            assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
+
+
+def test_version_and_partials_bytes():
+    assert "gfx950" in _lib.version()
+    assert _lib.partials_bytes(_lib.FS_F32, _lib.FS_ACC_NATIVE, 10, 1) == 80
+    assert _lib.partials_bytes(_lib.FS_C128, _lib.FS_ACC_NATIVE, 10, 4) == 640
+    assert _lib.partials_bytes(99, 0, 1, 1) == -1
+
+
+def _csr(**kw):
+    base = dict(value_type=_lib.FS_F32, index_type=_lib.FS_I32, ptr_type=_lib.FS_I32, reserved=0,
+                n_rows=4, n_cols=4, nnz=4, row_ptr=16, col_idx=32, values=48)
+    base.update(kw)
+    return _lib.FsCsr(**base)
+
+
+def _plan(**kw):
+    base = dict(accumulate=0, group_lanes=8, xcd_remap=0, unroll=4, short_max=64, chunk_nnz=512,
+                n_long=0, long_rows=None, long_chunk_ptr=None, n_chunks=0, chunk_long=None,
+                counters=None, partials=None, partials_bytes=0)
+    base.update(kw)
+    return _lib.FsPlan(**base)
+
+
+def _spmv(csr, plan):
+    lib = _lib.load()
+    return lib.fs_spmv_csr(ctypes.byref(csr) if csr is not None else None, 64, 80, 1.0, 0.0,
+                           ctypes.byref(plan) if plan is not None else None, None)
+
+
+@pytest.mark.parametrize("csr,plan,code,msg", [
+    (None, _plan(), _lib.FS_ERR_VALUE, "descriptor is null"),
+    (_csr(value_type=42), _plan(), _lib.FS_ERR_TYPE, "value_type"),
+    (_csr(index_type=7), _plan(), _lib.FS_ERR_TYPE, "FS_I32 or FS_I64"),
+    (_csr(n_rows=-1), _plan(), _lib.FS_ERR_VALUE, "negative"),
+    (_csr(row_ptr=None), _plan(), _lib.FS_ERR_VALUE, "row_ptr is null"),
+    (_csr(n_cols=2**31), _plan(), _lib.FS_ERR_VALUE, "int32 column indices"),
+    (_csr(), None, _lib.FS_ERR_VALUE, "plan is null"),
+    (_csr(), _plan(accumulate=5), _lib.FS_ERR_VALUE, "accumulate"),
+    (_csr(), _plan(short_max=-1), _lib.FS_ERR_VALUE, "short_max"),
+    (_csr(), _plan(n_long=1), _lib.FS_ERR_VALUE, "metadata pointers"),
+    (_csr(), _plan(n_long=1, long_rows=8, long_chunk_ptr=8, chunk_long=8, counters=8,
+                   n_chunks=1, chunk_nnz=100), _lib.FS_ERR_VALUE, "multiple of 64"),
+    (_csr(), _plan(n_long=1, long_rows=8, long_chunk_ptr=8, chunk_long=8, counters=8,
+                   n_chunks=1, chunk_nnz=64, partials=8, partials_bytes=4),
+     _lib.FS_ERR_VALUE, "partials buffer too small"),
+    (_csr(), _plan(n_chunks=3), _lib.FS_ERR_VALUE, "chunks listed"),
+])
+def test_spmv_rejects_bad_arguments(csr, plan, code, msg):
+    rc = _spmv(csr, plan)
+    assert rc == code
+    assert msg in _lib.load().fs_last_error().decode()
+
+
+def test_group_lanes_checked_before_launch():
+    # a bad group width is reported without touching the device (n_rows > 0,
+    # pointers are never dereferenced on the host)
+    rc = _spmv(_csr(), _plan(group_lanes=3))
+    assert rc == _lib.FS_ERR_VALUE
+    assert "group_lanes" in _lib.load().fs_last_error().decode()
+
+
+def test_status_to_exception_mapping():
+    lib = _lib.load()
+    lib.fs_spmv_csr(None, None, None, 1.0, 0.0, None, None)  # sets the message
+    with pytest.raises(ValueError):
+        _lib.check(_lib.FS_ERR_VALUE)
+    with pytest.raises(TypeError):
+        _lib.check(_lib.FS_ERR_TYPE)
+    with pytest.raises(IndexError):
+        _lib.check(_lib.FS_ERR_INDEX)
+    with pytest.raises(RuntimeError):
+        _lib.check(_lib.FS_ERR_HIP)
+    _lib.check(_lib.FS_OK)
+
+
+def test_spmm_and_analyze_reject_bad_arguments():
+    lib = _lib.load()
+    csr = _csr()
+    rc = lib.fs_spmm_csr(ctypes.byref(csr), 64, 4, 80, 4, -1, 1.0, 0.0, ctypes.byref(_plan()), 0, None)
+    assert rc == _lib.FS_ERR_VALUE and "n_dense" in lib.fs_last_error().decode()
+    rc = lib.fs_spmm_csr(ctypes.byref(csr), 64, 4, 80, 2, 4, 1.0, 0.0, ctypes.byref(_plan()), 0, None)
+    assert rc == _lib.FS_ERR_VALUE and "leading dimensions" in lib.fs_last_error().decode()
+    rc = lib.fs_csr_analyze(ctypes.byref(csr), None, None)
+    assert rc == _lib.FS_ERR_VALUE and "stats is null" in lib.fs_last_error().decode()
+    bad = _csr(ptr_type=3)
+    assert lib.fs_csr_analyze(ctypes.byref(bad), 8, None) == _lib.FS_ERR_TYPE
+
+
+def test_dtype_codes_cover_reference_tables():
+    import torch
+    from flagsparse_public_amd.sparse_operations._common import SUPPORTED_VALUE_DTYPES
+    from flagsparse_public_amd.sparse_operations.spmv_csr import SUPPORTED_SPMV_VALUE_DTYPES
+    for d in SUPPORTED_VALUE_DTYPES + SUPPORTED_SPMV_VALUE_DTYPES:
+        _lib.value_code(d)
+    with pytest.raises(TypeError):
+        _lib.value_code(torch.int8)

@@ -1,0 +1,289 @@
+"""GPU parity of the HIP CSR SpMV against the CPU oracle (MI355X).
+
+Bars (per row, against the fp64 oracle y64 and the condition sum|a x|):
+  * fp32 default path (fp64 accumulation, one rounding): bit-identical to
+    fl32(y64) on >= 99.9% of rows and within 1 fp32 ulp on all rows;
+  * fp32 opt path (native accumulation): |y - y64| <= 1e-5 * sum|a x|;
+  * fp64: |y - y64| <= 1e-12 * sum|a x|;
+  * complex64: 1e-5, complex128: 1e-12 (same measure on |.|);
+  * fp16 / bf16: the reference suite's allclose (5e-3 / 1e-1).
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+from tests._util import golden_case, host64, random_mask_csr, row_rel_err, to_dev, ulp_distance_f32
+
+pytestmark = pytest.mark.gpu
+
+spmv_mod = importlib.import_module("flagsparse_public_amd.sparse_operations.spmv_csr")
+from flagsparse_public_amd import flagsparse_spmv_csr, prepare_spmv_csr  # noqa: E402
+from flagsparse_public_amd.rowsplit import Schedule  # noqa: E402
+from flagsparse_public_amd import synthetic  # noqa: E402
+
+DEV = "cuda"
+
+
+def _oracle_parts(orc, data, indices, indptr, x):
+    d = host64(data) if not data.is_complex() else data.cpu().to(torch.complex128).numpy()
+    xx = host64(x) if not x.is_complex() else x.cpu().to(torch.complex128).numpy()
+    ip = indptr.cpu().numpy().astype(np.int64)
+    ix = indices.cpu().numpy().astype(np.int64)
+    return orc.spmv(ip, ix, d, xx), orc.spmv_abs(ip, ix, d, xx)
+
+
+def _check(y, y64, cond, dtype, path):
+    y = host64(y) if not y.is_complex() else y.cpu().to(torch.complex128).numpy()
+    if dtype == torch.float32 and path == "default":
+        ulps = ulp_distance_f32(y.astype(np.float32), y64.astype(np.float32))
+        assert ulps.max() <= 1, f"max ulp {ulps.max()}"
+        assert (ulps == 0).mean() >= 0.999
+    elif dtype in (torch.float32, torch.complex64):
+        assert row_rel_err(y, y64, cond).max() <= 1e-5
+    elif dtype in (torch.float64, torch.complex128):
+        assert row_rel_err(y, y64, cond).max() <= 1e-12
+    elif dtype == torch.float16:
+        assert np.allclose(y, y64, rtol=5e-3, atol=5e-3)
+    else:
+        assert np.allclose(y, y64, rtol=1e-1, atol=1e-1)
+
+
+@pytest.mark.parametrize("name", ["ka2x2", "longrow", "unsorted_dup", "empty_rows", "all_empty",
+                                  "cancel", "complex"])
+def test_spmv_golden_exact(name):
+    c = golden_case(name)
+    data = to_dev(c["data"])
+    indices = to_dev(c["indices"].astype(np.int32))
+    indptr = to_dev(c["indptr"])
+    x = to_dev(c["x"])
+    shape = tuple(int(v) for v in c["shape"])
+    y = flagsparse_spmv_csr(data, indices, indptr, x, shape)
+    if data.dtype == torch.float32:
+        # default path: fp64 accumulation rounded once == exactly rounded result
+        assert np.array_equal(y.cpu().numpy(), c["y_f32"])
+    else:
+        got = y.cpu().numpy()
+        cond = np.abs(c["y_exact"]) + 1e-300
+        assert np.all(np.abs(got - c["y_exact"]) <= 1e-15 * np.maximum(cond, 1.0))
+
+
+@pytest.mark.parametrize("M,N", [(1, 32), (160, 1024), (128, 256)])
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.float32, torch.float64,
+                                   torch.complex64, torch.complex128],
+                         ids=["float16", "bfloat16", "float32", "float64", "complex64", "complex128"])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["int32", "int64"])
+@pytest.mark.parametrize("op", ["non", "trans", "conj"])
+def test_spmv_matches_oracle_grid(orc, M, N, dtype, index_dtype, op):
+    """The reference accuracy grid (test_spmv_csr_accuracy.py:108-132), checked
+    against the fp64 oracle instead of a dense torch product."""
+    seed = M * 7919 + N
+    data, indices, indptr, dense = random_mask_csr(M, N, dtype, index_dtype, seed)
+    transpose = op in ("trans", "conj")
+    g = torch.Generator().manual_seed(seed + 1)
+    xlen = M if transpose else N
+    if dtype.is_complex:
+        comp = torch.float32 if dtype == torch.complex64 else torch.float64
+        x = torch.complex(torch.randn(xlen, generator=g, dtype=comp),
+                          torch.randn(xlen, generator=g, dtype=comp))
+    else:
+        x = torch.randn(xlen, generator=g).to(dtype)
+    ref_dense = dense.to(torch.complex128 if dtype.is_complex else torch.float64)
+    if op == "trans":
+        ref_dense = ref_dense.t()
+    elif op == "conj":
+        ref_dense = ref_dense.conj().t().resolve_conj()
+    A = ref_dense.numpy()
+    xx = x.to(torch.complex128 if dtype.is_complex else torch.float64).numpy()
+    y64 = A @ xx
+    cond = np.abs(A) @ np.abs(xx)
+    y = flagsparse_spmv_csr(data.to(DEV), indices.to(DEV), indptr.to(DEV), x.to(DEV), (M, N), op=op)
+    assert y.dtype == dtype and tuple(y.shape) == (N if transpose else M,)
+    _check(y, y64, cond, dtype, "default")
+    if dtype in (torch.float32, torch.float64) and index_dtype == torch.int32:
+        y2 = flagsparse_spmv_csr(data.to(DEV), indices.to(DEV), indptr.to(DEV), x.to(DEV), (M, N),
+                                 op=op, use_opt=True)
+        _check(y2, y64, cond, dtype, "opt")
+
+
+def _powerlaw(n, seed, max_len, dtype=np.float32):
+    return synthetic.powerlaw_csr(n, seed, alpha=1.6, max_len=max_len, dtype=dtype)
+
+
+SCHEDULES = [
+    Schedule(group_lanes=s, short_max=sm, chunk_nnz=ch, unroll=u, xcd_remap=xr)
+    for s, sm, ch, u, xr in [
+        (1, 16, 64, 4, 0), (2, 32, 128, 2, 1), (4, 64, 512, 4, 0), (8, 128, 512, 8, 1),
+        (16, 256, 1024, 4, 0), (32, 512, 2048, 4, 1), (64, 4096, 4096, 8, 0), (8, 0, 64, 4, 0),
+    ]
+]
+
+
+@pytest.mark.parametrize("sch", SCHEDULES, ids=lambda s: f"S{s.group_lanes}_m{s.short_max}_c{s.chunk_nnz}_u{s.unroll}_x{s.xcd_remap}")
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+@pytest.mark.parametrize("ptr_dtype", [torch.int32, torch.int64], ids=["p32", "p64"])
+def test_spmv_schedules_powerlaw(orc, sch, dtype, ptr_dtype):
+    npd = np.float32 if dtype == torch.float32 else np.float64
+    ip, cols, vals = _powerlaw(20000, 11, 30000, npd)
+    x = synthetic.dense_uniform(12, ip.size - 1, npd)
+    y64 = orc.spmv(ip, cols, vals, x)
+    cond = orc.spmv_abs(ip, cols, vals, x)
+    data, indices, indptr = to_dev(vals), to_dev(cols), to_dev(ip).to(ptr_dtype)
+    xd = to_dev(x)
+    n = ip.size - 1
+    p = prepare_spmv_csr(data, indices, indptr, (n, n), schedule=sch)
+    y_def = flagsparse_spmv_csr(x=xd, prepared=p)
+    _check(y_def, y64, cond, dtype, "default")
+    y_opt = flagsparse_spmv_csr(x=xd, prepared=p, use_opt=True)
+    _check(y_opt, y64, cond, dtype, "opt")
+    # deterministic: the chunked rows' fix-up sums partials in chunk order
+    for _ in range(3):
+        assert torch.equal(flagsparse_spmv_csr(x=xd, prepared=p, use_opt=True), y_opt)
+        assert torch.equal(flagsparse_spmv_csr(x=xd, prepared=p), y_def)
+
+
+def test_spmv_long_rows_compensated_fp32(orc):
+    """Rows far above 4096 entries with heavy cancellation stay within 1e-5 of
+    sum|a x| on the native fp32 path (compensated chunk accumulation)."""
+    rng = np.random.default_rng(5)
+    n, m = 8, 300000
+    lens = np.array([250000, 100000, 5000, 4097, 1, 0, 60000, 3])
+    ip = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    cols = np.concatenate([np.sort(rng.choice(m, l, replace=False)) for l in lens]).astype(np.int32)
+    vals = (rng.uniform(-1, 1, ip[-1]) * 1e3).astype(np.float32)
+    x = rng.uniform(-1, 1, m).astype(np.float32)
+    y64 = orc.spmv(ip, cols, vals, x)
+    cond = orc.spmv_abs(ip, cols, vals, x)
+    y = flagsparse_spmv_csr(to_dev(vals), to_dev(cols), to_dev(ip), to_dev(x), (n, m), use_opt=True)
+    assert row_rel_err(host64(y), y64, cond).max() <= 1e-6
+
+
+def test_spmv_alpha_beta_and_out():
+    ip, cols, vals = synthetic.poisson_csr(3000, 9.0, 3, lo=0)
+    x = synthetic.dense_uniform(4, 3000)
+    y0 = synthetic.dense_uniform(5, 3000)
+    data, indices, indptr, xd = to_dev(vals), to_dev(cols), to_dev(ip), to_dev(x)
+    base = flagsparse_spmv_csr(data, indices, indptr, xd, (3000, 3000), use_opt=True)
+    out = to_dev(y0)
+    res = flagsparse_spmv_csr(data, indices, indptr, xd, (3000, 3000), out=out, alpha=2.0,
+                              beta=-0.5, use_opt=True)
+    assert res is out
+    expect = 2.0 * host64(base) - 0.5 * y0.astype(np.float64)
+    assert np.allclose(host64(res), expect, rtol=1e-6, atol=1e-6)
+    out2 = torch.full((3000,), float("nan"), device=DEV)
+    res2 = flagsparse_spmv_csr(data, indices, indptr, xd, (3000, 3000), out=out2)
+    assert res2 is out2 and not torch.isnan(res2).any()
+
+
+def test_spmv_empty_shapes():
+    e = torch.empty(0, device=DEV)
+    y = flagsparse_spmv_csr(e, torch.empty(0, dtype=torch.int32, device=DEV),
+                            torch.zeros(1, dtype=torch.int64, device=DEV),
+                            torch.ones(5, device=DEV), (0, 5))
+    assert y.shape == (0,)
+    y = flagsparse_spmv_csr(e, torch.empty(0, dtype=torch.int32, device=DEV),
+                            torch.zeros(4, dtype=torch.int64, device=DEV),
+                            torch.ones(5, device=DEV), (3, 5))
+    assert torch.equal(y, torch.zeros(3, device=DEV))
+
+
+def test_spmv_16k_workload(orc):
+    """16384-row Poisson(16) SpMV, all paths."""
+    ip, cols, vals = synthetic.spmv_16k()
+    x = synthetic.dense_uniform(101, 16384)
+    y64 = orc.spmv(ip, cols, vals, x)
+    cond = orc.spmv_abs(ip, cols, vals, x)
+    args = (to_dev(vals), to_dev(cols), to_dev(ip), to_dev(x), (16384, 16384))
+    _check(flagsparse_spmv_csr(*args), y64, cond, torch.float32, "default")
+    _check(flagsparse_spmv_csr(*args, use_opt=True), y64, cond, torch.float32, "opt")
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+def test_spmv_powerlaw_4m_workload(orc, dtype):
+    """The 4M-row power-law SpMV at full size against the oracle."""
+    npd = np.float32 if dtype == torch.float32 else np.float64
+    ip, cols, vals = synthetic.spmv_powerlaw_4m(dtype=npd)
+    n = ip.size - 1
+    x = synthetic.dense_uniform(202, n, npd)
+    y64 = orc.spmv(ip, cols, vals, x)
+    cond = orc.spmv_abs(ip, cols, vals, x)
+    p = prepare_spmv_csr(to_dev(vals), to_dev(cols), to_dev(ip), (n, n))
+    xd = to_dev(x)
+    _check(flagsparse_spmv_csr(x=xd, prepared=p, use_opt=True), y64, cond, dtype, "opt")
+    if dtype == torch.float32:
+        _check(flagsparse_spmv_csr(x=xd, prepared=p), y64, cond, dtype, "default")
+
+
+# ---- the reference's own API / fallback tests (test_spmv_csr_accuracy.py:135-232)
+
+def test_prepared_transpose_mismatch_rejected():
+    data, indices, indptr, _ = random_mask_csr(8, 10, torch.float32, torch.int32, 1)
+    p = spmv_mod.prepare_spmv_csr(data.to(DEV), indices.to(DEV), indptr.to(DEV), (8, 10),
+                                  transpose=True)
+    with pytest.raises(ValueError, match="does not match prepared.transpose"):
+        flagsparse_spmv_csr(x=torch.randn(8, device=DEV), prepared=p, transpose=False)
+
+
+def test_prepared_op_mismatch_rejected():
+    data, indices, indptr, _ = random_mask_csr(8, 10, torch.complex64, torch.int32, 2)
+    p = spmv_mod.prepare_spmv_csr(data.to(DEV), indices.to(DEV), indptr.to(DEV), (8, 10), op="conj")
+    x = torch.randn(8, dtype=torch.complex64, device=DEV)
+    with pytest.raises(ValueError, match="does not match prepared.op"):
+        flagsparse_spmv_csr(x=x, prepared=p, op="trans")
+
+
+def test_int64_auto_fallback_to_int32(monkeypatch):
+    data, indices, indptr, dense = random_mask_csr(12, 9, torch.float32, torch.int64, 3)
+    x = torch.randn(9)
+    ref = dense.double() @ x.double()
+    state = {"forced_once": False}
+    original = spmv_mod._triton_spmv_csr_impl_prepared
+
+    def fail_int64_once(prepared, x_in):
+        if prepared.kernel_indices.dtype == torch.int64 and not state["forced_once"]:
+            state["forced_once"] = True
+            raise RuntimeError("forced int64 launch failure")
+        return original(prepared, x_in)
+
+    monkeypatch.setattr(spmv_mod, "_triton_spmv_csr_impl_prepared", fail_int64_once)
+    out = flagsparse_spmv_csr(data.to(DEV), indices.to(DEV), indptr.to(DEV), x.to(DEV), (12, 9),
+                              index_fallback_policy="auto")
+    assert state["forced_once"]
+    assert torch.allclose(out.double().cpu(), ref, rtol=1e-4, atol=1e-4)
+
+
+def test_int64_strict_no_fallback(monkeypatch):
+    data, indices, indptr, _ = random_mask_csr(12, 9, torch.float32, torch.int64, 4)
+
+    def fail_int64(prepared, x_in):
+        if prepared.kernel_indices.dtype == torch.int64:
+            raise RuntimeError("forced int64 launch failure")
+        return spmv_mod._hip_spmv_csr_impl_prepared(prepared, x_in)
+
+    monkeypatch.setattr(spmv_mod, "_triton_spmv_csr_impl_prepared", fail_int64)
+    with pytest.raises(RuntimeError, match="forced int64 launch failure"):
+        flagsparse_spmv_csr(data.to(DEV), indices.to(DEV), indptr.to(DEV),
+                            torch.randn(9, device=DEV), (12, 9), index_fallback_policy="strict")
+
+
+def test_spmv_validation_errors():
+    d = torch.tensor([1.0, 2.0, 3.0], device=DEV)
+    i = torch.tensor([0, 1, 1], dtype=torch.int32, device=DEV)
+    p = torch.tensor([0, 2, 3], dtype=torch.int64, device=DEV)
+    x = torch.ones(2, device=DEV)
+    with pytest.raises(ValueError, match="indptr must start at zero"):
+        flagsparse_spmv_csr(d, i, torch.tensor([1, 2, 3], device=DEV), x, (2, 2))
+    with pytest.raises(ValueError, match=r"indptr\[-1\] must equal nnz"):
+        flagsparse_spmv_csr(d, i, torch.tensor([0, 2, 2], device=DEV), x, (2, 2))
+    with pytest.raises(ValueError, match="non-decreasing"):
+        flagsparse_spmv_csr(d, i, torch.tensor([0, 3, 2], device=DEV), x, (2, 2))
+    with pytest.raises(IndexError):
+        flagsparse_spmv_csr(d, torch.tensor([0, 3, 1], dtype=torch.int32, device=DEV), p, x, (2, 2))
+    with pytest.raises(TypeError, match="x dtype"):
+        flagsparse_spmv_csr(d, i, p, x.double(), (2, 2))
+    with pytest.raises(ValueError, match="x length"):
+        flagsparse_spmv_csr(d, i, p, torch.ones(3, device=DEV), (2, 2))

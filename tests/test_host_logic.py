@@ -1,0 +1,255 @@
+"""Host-side logic of the operator modules (CPU only): op codes, validation
+order and messages, bucket / split / transpose metadata against the oracle,
+the int64 -> int32 fallback contract, schedules and the synthetic workloads."""
+
+import importlib
+
+import numpy as np
+import pytest
+import torch
+
+spmv_mod = importlib.import_module("flagsparse_public_amd.sparse_operations.spmv_csr")
+spmm_mod = importlib.import_module("flagsparse_public_amd.sparse_operations.spmm_csr")
+alg2_mod = importlib.import_module("flagsparse_public_amd.sparse_operations.spmm_csr_opt_alg2")
+from flagsparse_public_amd import rowsplit, synthetic  # noqa: E402
+
+
+@pytest.mark.parametrize("op,transpose,code", [
+    (None, False, 0), (None, True, 1), ("non", False, 0), ("TRANS", False, 1), (" conj ", False, 2),
+    (0, False, 0), (1, False, 1), (2, False, 2), (1.0, False, 1)])
+def test_normalize_op(op, transpose, code):
+    assert spmv_mod._normalize_spmv_op(op, transpose=transpose) == code
+
+
+@pytest.mark.parametrize("bad", ["herm", "1", 3, -1, object()])
+def test_normalize_op_rejects(bad):
+    with pytest.raises(ValueError, match="op must be one of: 0=non, 1=trans, 2=conj"):
+        spmv_mod._normalize_spmv_op(bad)
+
+
+def test_policy_and_conflicts():
+    with pytest.raises(ValueError, match="index_fallback_policy"):
+        spmv_mod._normalize_spmv_index_fallback_policy("lenient")
+    d = torch.ones(1)
+    with pytest.raises(ValueError, match="transpose=True conflicts with op=non"):
+        spmv_mod.prepare_spmv_csr(d, d.int(), torch.tensor([0, 1]), (1, 1), transpose=True, op="non")
+    with pytest.raises(ValueError, match="transpose conflicts with op"):
+        spmv_mod.flagsparse_spmv_csr(d, d.int(), torch.tensor([0, 1]), d, (1, 1), transpose=False,
+                                     op="trans")
+
+
+def test_validation_order_on_cpu_tensors():
+    """Checks that precede the device check fire first, with the reference's
+    messages (spmv_csr.py:497-524); CPU tensors then fail the device check."""
+    d = torch.ones(3)
+    i = torch.zeros(3, dtype=torch.int32)
+    p = torch.tensor([0, 3])
+    with pytest.raises(TypeError, match="must all be torch.Tensor"):
+        spmv_mod._prepare_spmv_csr_matrix([1.0], i, p, (1, 4))
+    with pytest.raises(ValueError, match="must be 1D tensors"):
+        spmv_mod._prepare_spmv_csr_matrix(d.view(3, 1), i, p, (1, 4))
+    with pytest.raises(ValueError, match=r"indptr length must be n_rows\+1=3, got 2"):
+        spmv_mod._prepare_spmv_csr_matrix(d, i, p, (2, 4))
+    with pytest.raises(ValueError, match="same length"):
+        spmv_mod._prepare_spmv_csr_matrix(d, i[:2], p, (1, 4))
+    with pytest.raises(ValueError, match="must be CUDA tensors"):
+        spmv_mod._prepare_spmv_csr_matrix(d, i, p, (1, 4))
+    with pytest.raises(ValueError, match="shape must be a 2-tuple"):
+        spmm_mod._prepare_spmm_csr_matrix(d, i, p, (1,))
+    with pytest.raises(ValueError, match="non-negative"):
+        spmm_mod._prepare_spmm_csr_matrix(d, i, torch.tensor([0]), (-1, 4))
+    with pytest.raises(ValueError, match="B must be a 2D"):
+        spmm_mod._prepare_spmm_csr_inputs(d, i, p, d, (1, 4))
+
+
+def test_spmv_x_validation():
+    class P:
+        data = torch.ones(2)
+        n_cols = 4
+    with pytest.raises(TypeError, match="x must be a torch.Tensor"):
+        spmv_mod._validate_spmv_x(None, P)
+    with pytest.raises(ValueError, match="x must be a 1D tensor"):
+        spmv_mod._validate_spmv_x(torch.ones(2, 2), P)
+    with pytest.raises(ValueError, match="x must be a CUDA tensor"):
+        spmv_mod._validate_spmv_x(torch.ones(4), P)
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_spmv_bucket_rows_match_oracle(orc, fp64):
+    lens = torch.tensor([0, 1, 64, 65, 300, 512, 513, 2048, 2049, 4096, 4097, 9000, 7])
+    buckets = spmv_mod._build_spmv_opt_buckets(lens, int(lens.max()), torch.int32, fp64=fp64)
+    got = np.full(lens.numel(), -1)
+    for b_i, b in enumerate(buckets):
+        got[b["rows"].numpy()] = b_i
+    want = orc.spmv_opt_bucket(lens.numpy(), fp64=fp64)
+    # bucket list skips empty buckets: compare the partition via sorted labels
+    nonempty = sorted(set(want.tolist()))
+    remap = {v: k for k, v in enumerate(nonempty)}
+    assert got.tolist() == [remap[v] for v in want.tolist()]
+    assert all(b["rows"].dtype == torch.int32 for b in buckets)
+
+
+def test_spmm_buckets_and_split_parts_match_oracle(orc):
+    lens = torch.tensor([0, 32, 33, 128, 129, 512, 513, 2048, 2049, 5000, 1])
+    buckets, long_rows = spmm_mod._build_spmm_opt_buckets(lens)
+    want = orc.spmm_opt_bucket(lens.numpy())
+    for b in buckets:
+        kinds = {"batched": (0, 1), "vector": (2, 3), "split": (4,)}[b["kind"]]
+        assert all(want[r] in kinds for r in b["rows"].tolist())
+    assert long_rows.tolist() == [8, 9]
+    indptr = torch.zeros(lens.numel() + 1, dtype=torch.int64)
+    indptr[1:] = torch.cumsum(lens, 0)
+    pr, ps, pe, rpp = spmm_mod._build_spmm_opt_split_metadata(indptr, long_rows, 256)
+    opr, ops, ope, orpp = orc.split_parts(indptr.numpy(), long_rows.numpy(), 256)
+    assert pr.tolist() == opr.tolist() and ps.tolist() == ops.tolist()
+    assert pe.tolist() == ope.tolist() and rpp.tolist() == orpp.tolist()
+
+
+def test_alg2_buckets_match_oracle(orc):
+    lens = torch.tensor([0, 16, 17, 64, 65, 256, 257, 1024, 1025, 3])
+    buckets = alg2_mod._build_spmm_opt_alg2_buckets(lens, torch.float32)
+    want = orc.spmm_alg2_bucket(lens.numpy())
+    labels = ["short_16", "short_64", "row_256", "seg_1024", "seg_long"]
+    for b in buckets:
+        assert all(labels[want[r]] == b["label"] for r in b["rows"].tolist())
+    assert sum(b["rows"].numel() for b in buckets) == lens.numel()
+
+
+def test_alg2_launch_resolution_on_wave64():
+    props = {"warp_size": 64, "max_threads_per_block": 1024, "max_threads_per_mp": 2048,
+             "shared_memory_per_block": 65536}
+    b = dict(alg2_mod._SPMM_OPT_ALG2_BUCKET_SPECS_F32[4])
+    b["rows"] = torch.arange(3)
+    launch = alg2_mod._resolve_spmm_opt_alg2_launch(b, 64, torch.float32, props)
+    assert launch["block_n"] == 64 and launch["num_warps"] == 16 and launch["num_stages"] == 1
+    assert alg2_mod._select_spmm_opt_alg2_block_n(5, 128) == 8
+    assert alg2_mod._select_spmm_opt_alg2_block_n(200, 32) == 32
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_transpose_matches_oracle(orc, seed):
+    rng = np.random.default_rng(seed)
+    n, m = 40, 30
+    lens = rng.poisson(3, n)
+    ip = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    cols = rng.integers(0, m, ip[-1])
+    vals = rng.normal(size=ip[-1])
+    d, idx, ptr, shape = spmv_mod._transpose_csr_for_spmv(
+        torch.from_numpy(vals), torch.from_numpy(cols).int(), torch.from_numpy(ip), (n, m))
+    t_ip, t_idx, perm = orc.csr_transpose(ip, cols, (n, m))
+    assert shape == (m, n)
+    assert ptr.tolist() == t_ip.tolist() and idx.tolist() == t_idx.tolist()
+    assert np.array_equal(d.numpy(), vals[perm])
+    assert idx.dtype == torch.int32 and ptr.dtype == torch.int64
+
+
+def _direct_prepared(kernel_indices, kernel_indptr, n_cols):
+    data = torch.ones(kernel_indices.numel())
+    return spmv_mod.PreparedCsrSpmv(
+        data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr,
+        shape=(kernel_indptr.numel() - 1, n_cols), n_rows=kernel_indptr.numel() - 1, n_cols=n_cols,
+        block_nnz=256, max_segments=1, max_row_nnz=1, opt_buckets=[], transpose=False,
+        index_fallback_policy="auto")
+
+
+def test_fallback_refused_when_index_exceeds_int32(monkeypatch):
+    """test_spmv_csr_accuracy.py:206-232, on the host."""
+    limit = spmv_mod._INDEX_LIMIT_INT32
+    p = _direct_prepared(torch.tensor([limit + 1]), torch.tensor([0, 1]), limit + 2)
+
+    def fail(_p, _x, use_opt=False):
+        raise RuntimeError("forced native int64 failure")
+
+    monkeypatch.setattr(spmv_mod, "_run_spmv_prepared", fail)
+    with pytest.raises(RuntimeError, match="int32 fallback is unsafe"):
+        spmv_mod._run_spmv_prepared_with_fallback(p, torch.empty(0), use_opt=False)
+
+
+def test_fallback_retries_with_int32(monkeypatch):
+    p = _direct_prepared(torch.tensor([0, 2, 1]), torch.tensor([0, 2, 3]), 3)
+    seen = []
+
+    def launcher(prepared, x):
+        seen.append(prepared.kernel_indices.dtype)
+        if prepared.kernel_indices.dtype == torch.int64:
+            raise RuntimeError("forced int64 launch failure")
+        return "ok"
+
+    monkeypatch.setattr(spmv_mod, "_triton_spmv_csr_impl_prepared", launcher)
+    assert spmv_mod._run_spmv_prepared_with_fallback(p, None) == "ok"
+    assert seen == [torch.int64, torch.int32]
+    p.index_fallback_policy = "strict"
+    seen.clear()
+    with pytest.raises(RuntimeError, match="forced int64 launch failure"):
+        spmv_mod._run_spmv_prepared_with_fallback(p, None)
+
+
+def test_fallback_blocker_messages():
+    limit = spmv_mod._INDEX_LIMIT_INT32
+    p = _direct_prepared(torch.tensor([0]), torch.tensor([0, limit + 5], dtype=torch.int64), 2)
+    assert "nnz offset" in spmv_mod._spmv_int32_fallback_blocker(p)
+    p = _direct_prepared(torch.tensor([-1]), torch.tensor([0, 1]), 2)
+    assert "column index range" in spmv_mod._spmv_int32_fallback_blocker(p)
+
+
+def test_alg1_launch_config():
+    cfg = spmm_mod._resolve_spmm_alg1_launch_config(96, 1540)
+    assert (cfg["block_n"], cfg["block_nnz"], cfg["required_segments"]) == (128, 32, 49)
+    assert spmm_mod._select_spmm_alg1_warp_and_factor(5) == (8, 1)
+    with pytest.raises(ValueError, match="explicit max_segments"):
+        spmm_mod._resolve_spmm_alg1_launch_config(32, 1540, block_nnz=128, max_segments=4)
+    assert spmm_mod._select_spmm_opt_block_n(33) == 64
+
+
+def test_error_metrics():
+    ref = torch.tensor([1.0, -2.0, 4.0], dtype=torch.float64)
+    cand = torch.tensor([1.0, -2.5, 4.0], dtype=torch.float64)
+    m = spmm_mod._spmm_validation_metrics(cand, ref)
+    assert m["max_abs_error"] == 0.5 and m["max_relative_error"] == pytest.approx(0.125)
+    assert m["sum_relative_error"] == pytest.approx(0.5 / 7.0)
+    assert m["matches_threshold"] is False
+    assert spmm_mod._spmm_opt_reference_error(cand, ref, torch.float64) == pytest.approx(
+        0.5 / (1e-12 + 1e-10 * 2.0))
+
+
+def test_default_schedules():
+    s = rowsplit.default_spmv_schedule(4_000_000, 48_000_000)
+    assert s.group_lanes == 8 and s.short_max == 128 and s.chunk_nnz % 64 == 0
+    assert rowsplit.default_spmv_schedule(10, 10).group_lanes == 1
+    assert rowsplit.default_spmv_schedule(10, 100000).group_lanes == 64
+
+
+def test_synthetic_generators_deterministic_and_sorted():
+    a = synthetic.poisson_csr(2000, 12.0, 9)
+    b = synthetic.poisson_csr(2000, 12.0, 9)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+    ip, cols, vals = a
+    lens = np.diff(ip)
+    rows = np.repeat(np.arange(lens.size), lens)
+    key = rows * 2000 + cols.astype(np.int64)
+    assert np.all(np.diff(key) > 0)  # sorted, no duplicates within a row
+    assert lens.min() >= 1 and np.all(np.abs(vals) <= 1)
+    ip, cols, _ = synthetic.banded_random_csr(5000, 3)
+    lens = np.diff(ip)
+    assert lens.max() == 15 and lens.min() == 12
+    rows = np.repeat(np.arange(5000), lens)
+    band = np.abs(cols.astype(np.int64) - rows) <= 3
+    # band entries per row equal the clipped band width; the rest are off-band
+    assert np.array_equal(np.bincount(rows, weights=band, minlength=5000).astype(int),
+                          lens - 8)
+    ip, cols, _ = synthetic.block_diag_csr(20, 0)
+    assert np.array_equal(np.diff(ip), [8] * 16 + [4] * 4)
+    ip, cols, _ = synthetic.banded_csr(10, 0, 2)
+    assert np.array_equal(np.diff(ip), [3, 4, 5, 5, 5, 5, 5, 5, 4, 3])
+
+
+def test_powerlaw_lengths_distribution():
+    rng = np.random.Generator(np.random.PCG64(2))
+    l = synthetic.powerlaw_lengths(rng, 200000, 2.0, 1, 100000)
+    # continuous density ~ l^-2 on [1, 1e5], ceiling: mean = ln(1e5)/(1-1e-5) + ~0.577
+    assert abs(l.mean() - (np.log(1e5) / (1 - 1e-5) + 0.577)) < 0.6
+    assert l.min() >= 1 and l.max() <= 100000
+    # P(len > 128) = (1/128 - 1/1e5) / (1 - 1/1e5) for the ceiling of l > 128
+    assert abs((l > 128).mean() - (1 / 128 - 1e-5)) < 1e-3
