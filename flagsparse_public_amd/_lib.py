@@ -25,6 +25,7 @@ FS_OK, FS_ERR_VALUE, FS_ERR_TYPE, FS_ERR_INDEX, FS_ERR_HIP, FS_ERR_UNSUPPORTED =
 FS_F16, FS_BF16, FS_F32, FS_F64, FS_C32, FS_C64, FS_C128 = range(7)
 FS_I32, FS_I64 = 0, 1
 FS_ACC_NATIVE, FS_ACC_WIDE = 0, 1
+FS_STORE_BF16X2, FS_STORE_BF16, FS_STORE_F16_ROWSCALE = 1, 2, 3
 FS_STATS_LEN = 16
 
 _COMPLEX32 = getattr(torch, "complex32", None) or getattr(torch, "chalf", None)
@@ -76,6 +77,31 @@ class FsPlan(ctypes.Structure):
     ]
 
 
+class FsBatchEntry(ctypes.Structure):
+    _fields_ = [
+        ("A", FsCsr),
+        ("x", ctypes.c_void_p),
+        ("y", ctypes.c_void_p),
+        ("B", ctypes.c_void_p),
+        ("C", ctypes.c_void_p),
+        ("n_dense", ctypes.c_int64),
+        ("ldb", ctypes.c_int64),
+        ("ldc", ctypes.c_int64),
+        ("group_lanes", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class FsMlp(ctypes.Structure):
+    _fields_ = [
+        ("in_dim", ctypes.c_int32),
+        ("hidden", ctypes.c_int32),
+        ("out_dim", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("params", ctypes.c_void_p),
+    ]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -90,6 +116,33 @@ _SIGNATURES = {
                                    ctypes.c_double, ctypes.c_double, ctypes.POINTER(FsPlan),
                                    ctypes.c_int32, ctypes.c_void_p]),
     "fs_csr_analyze": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_void_p]),
+    "fs_split_values": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(FsCsr), ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fs_spmv_csr_split": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                         ctypes.POINTER(FsPlan), ctypes.c_void_p]),
+    "fs_batch_matrix_bytes": (ctypes.c_int64, []),
+    "fs_batch_pack": (ctypes.c_int, [ctypes.POINTER(FsBatchEntry), ctypes.c_int64, ctypes.c_void_p]),
+    "fs_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_void_p]),
+    "fs_mlp_param_count": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "fs_csr_features": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    "fs_mlp_forward": (ctypes.c_int, [ctypes.POINTER(FsMlp), ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
+    "fs_mlp_workspace_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.c_int64]),
+    "fs_mlp_backward": (ctypes.c_int, [ctypes.POINTER(FsMlp)] + [ctypes.c_void_p] * 6
+                        + [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_int64, ctypes.c_void_p]),
+    "fs_adam_step": (ctypes.c_int, [ctypes.c_void_p] * 4 + [ctypes.c_int64, ctypes.c_double,
+                                                            ctypes.c_double, ctypes.c_double,
+                                                            ctypes.c_double, ctypes.c_int64,
+                                                            ctypes.c_void_p]),
+    "fs_mlp_select": (ctypes.c_int, [ctypes.POINTER(FsMlp)] + [ctypes.c_void_p] * 6),
     "fs_last_error": (ctypes.c_char_p, []),
     "fs_version": (ctypes.c_char_p, []),
 }
@@ -176,14 +229,68 @@ def stream_handle(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
-def analyze(csr: FsCsr, device) -> list[int]:
-    """Run fs_csr_analyze and copy the FS_STATS_LEN statistics to the host."""
+def analyze_device(csr: FsCsr, device):
+    """Run fs_csr_analyze; returns the FS_STATS_LEN statistics on the device."""
     lib = load()
     stats = torch.empty(FS_STATS_LEN, dtype=torch.int64, device=device)
     with torch.cuda.device(device):
         check(lib.fs_csr_analyze(ctypes.byref(csr), stats.data_ptr(), stream_handle(device)),
               "fs_csr_analyze")
-    return stats.cpu().tolist()
+    return stats
+
+
+def analyze(csr: FsCsr, device) -> list[int]:
+    """Run fs_csr_analyze and copy the FS_STATS_LEN statistics to the host."""
+    return analyze_device(csr, device).cpu().tolist()
+
+
+def features(stats, n_rows: int, n_cols: int, nnz: int, device):
+    """16 selector features (device float32) from device statistics."""
+    f = torch.empty(16, dtype=torch.float32, device=device)
+    with torch.cuda.device(device):
+        check(load().fs_csr_features(stats.data_ptr(), int(n_rows), int(n_cols), int(nnz),
+                                     f.data_ptr(), stream_handle(device)), "fs_csr_features")
+    return f
+
+
+def mlp_struct(params, in_dim: int, hidden: int, out_dim: int) -> FsMlp:
+    return FsMlp(in_dim=int(in_dim), hidden=int(hidden), out_dim=int(out_dim), reserved=0,
+                 params=params.data_ptr())
+
+
+def mlp_forward(net: FsMlp, X, H1, H2, Y, device) -> None:
+    with torch.cuda.device(device):
+        check(load().fs_mlp_forward(ctypes.byref(net), ptr(X), int(X.shape[0]), ptr(H1), ptr(H2),
+                                    ptr(Y), stream_handle(device)), "fs_mlp_forward")
+
+
+def mlp_backward(net: FsMlp, X, H1, H2, Y, T, M, grad, loss, workspace, device) -> None:
+    with torch.cuda.device(device):
+        check(load().fs_mlp_backward(ctypes.byref(net), ptr(X), ptr(H1), ptr(H2), ptr(Y), ptr(T),
+                                     ptr(M), int(X.shape[0]), ptr(grad), ptr(loss), ptr(workspace),
+                                     int(workspace.numel()), stream_handle(device)),
+              "fs_mlp_backward")
+
+
+def mlp_workspace_bytes(in_dim, hidden, out_dim, batch) -> int:
+    return int(load().fs_mlp_workspace_bytes(in_dim, hidden, out_dim, batch))
+
+
+def mlp_param_count(in_dim, hidden, out_dim) -> int:
+    return int(load().fs_mlp_param_count(in_dim, hidden, out_dim))
+
+
+def adam_step(params, grad, m, v, lr, beta1, beta2, eps, step, device) -> None:
+    with torch.cuda.device(device):
+        check(load().fs_adam_step(ptr(params), ptr(grad), ptr(m), ptr(v), int(params.numel()),
+                                  float(lr), float(beta1), float(beta2), float(eps), int(step),
+                                  stream_handle(device)), "fs_adam_step")
+
+
+def mlp_select(net: FsMlp, feats, mu, sd, choice, scores, device) -> None:
+    with torch.cuda.device(device):
+        check(load().fs_mlp_select(ctypes.byref(net), ptr(feats), ptr(mu), ptr(sd), ptr(choice),
+                                   ptr(scores), stream_handle(device)), "fs_mlp_select")
 
 
 def spmv(csr: FsCsr, x, y, alpha: float, beta: float, plan: FsPlan, device) -> None:
@@ -200,6 +307,22 @@ def spmm(csr: FsCsr, B, ldb: int, C, ldc: int, n_dense: int, alpha: float, beta:
         check(lib.fs_spmm_csr(ctypes.byref(csr), ptr(B), int(ldb), ptr(C), int(ldc),
                               int(n_dense), float(alpha), float(beta), ctypes.byref(plan),
                               int(col_tile), stream_handle(device)), "fs_spmm_csr")
+
+
+def split_values(fmt: int, csr: FsCsr, hi, lo, row_scale, device) -> None:
+    lib = load()
+    with torch.cuda.device(device):
+        check(lib.fs_split_values(int(fmt), ctypes.byref(csr), ptr(hi), ptr(lo), ptr(row_scale),
+                                  stream_handle(device)), "fs_split_values")
+
+
+def spmv_split(csr: FsCsr, fmt: int, hi, lo, row_scale, x, y, alpha: float, beta: float,
+               plan: FsPlan, device) -> None:
+    lib = load()
+    with torch.cuda.device(device):
+        check(lib.fs_spmv_csr_split(ctypes.byref(csr), int(fmt), ptr(hi), ptr(lo), ptr(row_scale),
+                                    ptr(x), ptr(y), float(alpha), float(beta), ctypes.byref(plan),
+                                    stream_handle(device)), "fs_spmv_csr_split")
 
 
 def partials_bytes(value_type: int, accumulate: int, n_chunks: int, n_dense: int = 1) -> int:

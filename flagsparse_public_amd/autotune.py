@@ -1,0 +1,158 @@
+"""Exhaustive schedule autotuning and its timing database.
+
+The variant space is a fixed, ordered list of ``Schedule`` objects per
+operation (SpMV, SpMM with 64 dense columns).  ``tune_spmv`` / ``tune_spmm``
+time every variant on one prepared matrix with HIP events (median of
+``reps`` timed launches after ``warmup``) and return the times; the
+smallest is the exhaustive choice.  The learned selector is trained on
+these timings and judged against them (its regret is time(selected) /
+time(exhaustive best)).
+
+``TimingDB`` keeps (features, per-variant times) rows and saves them as an
+``.npz`` keyed by device name and variant-list version, so collection can
+resume.
+"""
+
+from __future__ import annotations
+
+import os
+import statistics
+
+import numpy as np
+import torch
+
+from .rowsplit import Schedule
+
+VARIANTS_VERSION = 1
+
+# SpMV (fp32, native accumulation) variants: group width, short-row bound,
+# chunk size, unroll, XCD remap.  Index 0..6 sweep the group width with the
+# default rule's bounds; the rest vary depth, chunking and locality.
+SPMV_VARIANTS = (
+    Schedule(group_lanes=1, short_max=64, chunk_nnz=512, unroll=4),
+    Schedule(group_lanes=2, short_max=64, chunk_nnz=512, unroll=4),
+    Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=4),
+    Schedule(group_lanes=8, short_max=128, chunk_nnz=512, unroll=4),
+    Schedule(group_lanes=16, short_max=256, chunk_nnz=512, unroll=4),
+    Schedule(group_lanes=32, short_max=512, chunk_nnz=1024, unroll=4),
+    Schedule(group_lanes=64, short_max=1024, chunk_nnz=2048, unroll=4),
+    Schedule(group_lanes=2, short_max=32, chunk_nnz=256, unroll=2),
+    Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=8),
+    Schedule(group_lanes=8, short_max=128, chunk_nnz=512, unroll=8),
+    Schedule(group_lanes=8, short_max=64, chunk_nnz=256, unroll=4),
+    Schedule(group_lanes=16, short_max=1024, chunk_nnz=2048, unroll=8),
+    Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=4, xcd_remap=1),
+    Schedule(group_lanes=8, short_max=128, chunk_nnz=512, unroll=4, xcd_remap=1),
+    Schedule(group_lanes=16, short_max=256, chunk_nnz=512, unroll=4, xcd_remap=1),
+    Schedule(group_lanes=32, short_max=4096, chunk_nnz=4096, unroll=8, xcd_remap=1),
+)
+# The fixed "plain default-tile" SpMV schedule the selector is compared with.
+SPMV_DEFAULT_INDEX = 4
+
+# SpMM (64 dense columns) variants: column tile (lane-group width), short-row
+# bound, chunk size, XCD remap.
+SPMM_VARIANTS = (
+    Schedule(short_max=256, chunk_nnz=256, col_tile=64),
+    Schedule(short_max=256, chunk_nnz=256, col_tile=32),
+    Schedule(short_max=256, chunk_nnz=256, col_tile=16),
+    Schedule(short_max=64, chunk_nnz=128, col_tile=64),
+    Schedule(short_max=1024, chunk_nnz=1024, col_tile=64),
+    Schedule(short_max=256, chunk_nnz=256, col_tile=64, xcd_remap=1),
+    Schedule(short_max=256, chunk_nnz=256, col_tile=32, xcd_remap=1),
+    Schedule(short_max=64, chunk_nnz=128, col_tile=32, xcd_remap=1),
+)
+SPMM_DEFAULT_INDEX = 0
+
+
+def time_launches(fn, reps: int = 7, warmup: int = 2, inner: int = 1) -> float:
+    """Median microseconds per call of fn() over reps event-timed groups of
+    ``inner`` calls on the current stream."""
+    for _ in range(warmup):
+        fn()
+    stream = torch.cuda.current_stream()
+    samples = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(inner):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        samples.append(a.elapsed_time(b) * 1000.0 / inner)
+    return float(statistics.median(samples))
+
+
+def tune_spmv(prepared, x, variants=SPMV_VARIANTS, reps=7, warmup=2, inner=1):
+    """Median us of every SpMV variant (native accumulation) on one matrix."""
+    from .sparse_operations.spmv_csr import _hip_spmv_csr_impl_opt_prepared
+    y = torch.empty(prepared.n_rows, dtype=prepared.data.dtype, device=prepared.data.device)
+    times = []
+    for sch in variants:
+        times.append(time_launches(
+            lambda s=sch: _hip_spmv_csr_impl_opt_prepared(prepared, x, out=y, schedule=s),
+            reps, warmup, inner))
+    return np.asarray(times)
+
+
+def tune_spmm(prepared, B, variants=SPMM_VARIANTS, reps=5, warmup=1, inner=1):
+    """Median us of every SpMM variant (native accumulation) on one matrix."""
+    from .sparse_operations.spmm_csr import _hip_spmm_csr_impl_opt_prepared
+    C = torch.empty((prepared.n_rows, B.shape[1]), dtype=B.dtype, device=B.device)
+    times = []
+    for sch in variants:
+        times.append(time_launches(
+            lambda s=sch: _hip_spmm_csr_impl_opt_prepared(prepared, B, out=C, schedule=s),
+            reps, warmup, inner))
+    return np.asarray(times)
+
+
+def best_schedule(times, variants):
+    i = int(np.argmin(times))
+    return i, variants[i]
+
+
+class TimingDB:
+    """Rows of (matrix id, family, features[16], times[K]) for one operation."""
+
+    def __init__(self, op: str, n_variants: int, device_name: str = ""):
+        self.op = op
+        self.n_variants = n_variants
+        self.device_name = device_name
+        self.ids, self.families, self.features, self.times, self.sizes = [], [], [], [], []
+
+    def add(self, mid: int, family: str, feats, times, n_rows: int, nnz: int):
+        self.ids.append(int(mid))
+        self.families.append(family)
+        self.features.append(np.asarray(feats, np.float64))
+        self.times.append(np.asarray(times, np.float64))
+        self.sizes.append((int(n_rows), int(nnz)))
+
+    def __len__(self):
+        return len(self.ids)
+
+    def arrays(self):
+        return (np.asarray(self.ids, np.int64), np.asarray(self.families),
+                np.asarray(self.features).reshape(-1, 16),
+                np.asarray(self.times).reshape(-1, self.n_variants),
+                np.asarray(self.sizes, np.int64).reshape(-1, 2))
+
+    def save(self, path: str):
+        ids, fam, feats, times, sizes = self.arrays()
+        np.savez_compressed(path, op=self.op, device_name=self.device_name,
+                            version=VARIANTS_VERSION, ids=ids, families=fam, features=feats,
+                            times=times, sizes=sizes)
+
+    @classmethod
+    def load(cls, path: str) -> "TimingDB":
+        z = np.load(path, allow_pickle=False)
+        db = cls(str(z["op"]), int(z["times"].shape[1]), str(z["device_name"]))
+        if int(z["version"]) != VARIANTS_VERSION:
+            raise ValueError(f"{path}: variant list version {int(z['version'])} != {VARIANTS_VERSION}")
+        for i, f, ft, t, sz in zip(z["ids"], z["families"], z["features"], z["times"], z["sizes"]):
+            db.add(int(i), str(f), ft, t, int(sz[0]), int(sz[1]))
+        return db
+
+
+def default_db_dir() -> str:
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_models")

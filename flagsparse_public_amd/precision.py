@@ -1,0 +1,144 @@
+"""Split-precision value storage for fp32 CSR SpMV.
+
+``prepare_spmv_csr_split`` validates an fp32 CSR matrix like
+``prepare_spmv_csr`` and re-encodes its values on the device into one of:
+
+* ``"bf16x2"``        -- hi = bf16(v), lo = bf16(v - hi); value = hi + lo
+                         (about 16 significant bits; 4 bytes / entry);
+* ``"bf16"``          -- the hi plane only (8 significant bits; 2 bytes);
+* ``"f16_rowscale"``  -- fp16(v / s_r) with s_r the power of two at or above
+                         the row's max |v| (11 bits; 2 bytes + 4 per row).
+
+``flagsparse_spmv_csr_split`` then computes y = alpha A x + beta y with fp32
+x / y and fp32 accumulation (compensated TwoSum summation for rows in the
+chunked long-row role, which holds every row over 4096 entries).  The
+reference stores values only in the call's dtype; these modes have no
+counterpart there and are measured beside the plain fp32 kernel.
+"""
+
+from __future__ import annotations
+
+import time
+
+import torch
+
+from . import _lib
+from .rowsplit import RowSplitPlan, Schedule, default_spmv_schedule
+from .sparse_operations.spmv_csr import _prepare_spmv_csr_matrix, _validate_spmv_x
+
+FORMATS = {
+    "bf16x2": _lib.FS_STORE_BF16X2,
+    "bf16": _lib.FS_STORE_BF16,
+    "f16_rowscale": _lib.FS_STORE_F16_ROWSCALE,
+}
+# value bytes per entry and per row of each format (for the byte model)
+FORMAT_BYTES = {"bf16x2": (4, 0), "bf16": (2, 0), "f16_rowscale": (2, 4)}
+
+
+class PreparedCsrSpmvSplit:
+    """An fp32 CSR matrix with values held in a split-precision format."""
+
+    def __init__(self, kernel_indices, kernel_indptr, shape, value_format, hi, lo, row_scale,
+                 row_lengths, max_row_nnz, schedule):
+        self.kernel_indices = kernel_indices
+        self.kernel_indptr = kernel_indptr
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.n_rows, self.n_cols = self.shape
+        self.value_format = value_format
+        self.hi, self.lo, self.row_scale = hi, lo, row_scale
+        self.row_lengths = row_lengths
+        self.max_row_nnz = max_row_nnz
+        self.schedule = schedule
+        self.data = torch.empty(0, dtype=torch.float32, device=kernel_indptr.device)  # x / y dtype
+        self._plans = {}
+        nnz = kernel_indices.numel()
+        self._csr = _lib.FsCsr(value_type=_lib.FS_F32, index_type=_lib.FS_I32,
+                               ptr_type=_lib.index_code(kernel_indptr.dtype), reserved=0,
+                               n_rows=self.n_rows, n_cols=self.n_cols, nnz=nnz,
+                               row_ptr=_lib.ptr(kernel_indptr), col_idx=_lib.ptr(kernel_indices),
+                               values=None)
+
+    def plan(self, schedule=None):
+        sch = schedule or self.schedule
+        p = self._plans.get(sch.key())
+        if p is None:
+            p = RowSplitPlan(self.kernel_indptr, self.n_rows, torch.float32, sch,
+                             _lib.FS_ACC_NATIVE, row_lengths=self.row_lengths)
+            self._plans[sch.key()] = p
+        return p
+
+    def value_bytes(self) -> int:
+        per_entry, per_row = FORMAT_BYTES[self.value_format]
+        return per_entry * self.kernel_indices.numel() + per_row * self.n_rows
+
+
+def prepare_spmv_csr_split(data, indices, indptr, shape, value_format="bf16x2", schedule=None):
+    """Validate (same checks as prepare_spmv_csr) and encode the values."""
+    if value_format not in FORMATS:
+        raise ValueError(f"value_format must be one of {sorted(FORMATS)}")
+    if torch.is_tensor(data) and data.dtype != torch.float32:
+        raise TypeError("split-precision storage is for float32 matrices")
+    (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
+     max_row_nnz) = _prepare_spmv_csr_matrix(data, indices, indptr, shape)
+    if kernel_indices.dtype != torch.int32:
+        if n_cols - 1 > 2**31 - 1:
+            raise ValueError("split-precision storage needs column indices that fit int32")
+        kernel_indices = kernel_indices.to(torch.int32)
+    dev = data.device
+    nnz = data.numel()
+    fmt = FORMATS[value_format]
+    hi = torch.empty(nnz, dtype=torch.int16, device=dev)
+    lo = torch.empty(nnz, dtype=torch.int16, device=dev) if value_format == "bf16x2" else None
+    scale = (torch.empty(n_rows, dtype=torch.float32, device=dev)
+             if value_format == "f16_rowscale" else None)
+    csr = _lib.make_csr(data, kernel_indices, kernel_indptr, n_rows, n_cols)
+    _lib.split_values(fmt, csr, hi, lo, scale, dev)
+    sch = schedule or default_spmv_schedule(n_rows, nnz)
+    return PreparedCsrSpmvSplit(kernel_indices, kernel_indptr, (n_rows, n_cols), value_format,
+                                hi, lo, scale, row_lengths, max_row_nnz, sch)
+
+
+def decoded_values(prepared) -> torch.Tensor:
+    """The fp32 values the kernels multiply with (for error reporting)."""
+    fmt = prepared.value_format
+    hi = prepared.hi
+    if fmt == "f16_rowscale":
+        h = hi.view(torch.float16).float()
+        return h * torch.repeat_interleave(prepared.row_scale, prepared.row_lengths)
+    v = (hi.to(torch.int32) << 16).view(torch.float32)
+    if fmt == "bf16x2":
+        v = v.double() + (prepared.lo.to(torch.int32) << 16).view(torch.float32).double()
+    return v.double()
+
+
+def flagsparse_spmv_csr_split(x, prepared, out=None, alpha=1.0, beta=0.0, return_time=False,
+                              schedule=None):
+    """y = alpha A x + beta y with split-precision values (see module doc)."""
+    x = _validate_spmv_x(x, prepared)
+    if out is not None:
+        if not out.is_cuda or out.device != x.device:
+            raise ValueError("out must be a CUDA tensor on the same device")
+        if tuple(out.shape) != (prepared.n_rows,) or out.dtype != torch.float32:
+            raise ValueError("out shape/dtype must match result")
+        if not out.is_contiguous():
+            raise ValueError("out must be contiguous")
+    if float(beta) != 0.0 and out is None:
+        raise ValueError("beta != 0 requires out")
+    y = out if out is not None else torch.empty(prepared.n_rows, dtype=torch.float32,
+                                                 device=x.device)
+    t0 = None
+    if return_time:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+    if prepared.n_rows:
+        plan = prepared.plan(schedule)
+        _lib.spmv_split(prepared._csr, FORMATS[prepared.value_format], prepared.hi, prepared.lo,
+                        prepared.row_scale, x, y, alpha, beta, plan.struct(1), x.device)
+    if return_time:
+        torch.cuda.synchronize()
+        return y, (time.perf_counter() - t0) * 1000.0
+    return y
+
+
+__all__ = ["FORMATS", "PreparedCsrSpmvSplit", "prepare_spmv_csr_split",
+           "flagsparse_spmv_csr_split", "decoded_values", "Schedule"]

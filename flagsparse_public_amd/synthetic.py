@@ -50,6 +50,15 @@ def unique_sorted_columns(rng, lengths: np.ndarray, n_cols: int) -> np.ndarray:
         return np.zeros(0, dtype=np.int64)
     rows = np.repeat(np.arange(n, dtype=np.int64), lengths)
     keys = rows * n_cols + rng.integers(0, n_cols, nnz, dtype=np.int64)
+    # rows denser than 1/8 draw without replacement directly (redrawing
+    # duplicates would take ~n_cols rounds for a nearly full row)
+    dense = np.nonzero(lengths * 8 > n_cols)[0]
+    if dense.size:
+        starts = np.zeros(n + 1, dtype=np.int64)
+        np.cumsum(lengths, out=starts[1:])
+        for r in dense:
+            keys[starts[r]:starts[r + 1]] = r * n_cols + rng.choice(n_cols, int(lengths[r]),
+                                                                    replace=False)
     keys.sort()
     while True:
         dup = np.zeros(nnz, dtype=bool)
@@ -198,3 +207,102 @@ def spmv_powerlaw_4m(seed=2, dtype=np.float32, n=4_000_000):
 def spmm_banded_1m(seed=3, dtype=np.float32, n=1_048_576):
     """n = 1,048,576, band -3..+3 plus 8 off-band columns."""
     return banded_random_csr(n, seed, half_band=3, n_random=8, dtype=dtype)
+
+
+# Device-side generators (torch, seeded philox on the GPU) for the selector's
+# matrix families.  Columns are uniform with replacement (duplicates summed as
+# stored) except where the family fixes them; rows are sorted by column.
+
+SELECTOR_FAMILIES = ("poisson", "powerlaw", "banded", "blockdiag", "rmat")
+
+
+def _torch_rows_to_csr(torch, rows_len, cols, n, vals_gen, device):
+    indptr = torch.zeros(n + 1, dtype=torch.int64, device=device)
+    torch.cumsum(rows_len, 0, out=indptr[1:])
+    nnz = int(indptr[-1].item())
+    vals = (torch.rand(nnz, generator=vals_gen, device=device) * 2 - 1).float()
+    return indptr, cols.to(torch.int32), vals
+
+
+def _torch_sorted_random_cols(torch, lens, n_cols, gen, device):
+    n = lens.numel()
+    nnz = int(lens.sum().item())
+    rows = torch.repeat_interleave(torch.arange(n, device=device), lens)
+    cols = torch.randint(0, n_cols, (nnz,), generator=gen, device=device)
+    key = rows * n_cols + cols
+    key, _ = torch.sort(key)
+    return key - rows * n_cols
+
+
+def family_csr_torch(family, n, seed, device, params=None):
+    """One matrix of a selector family on the device; returns (indptr int64,
+    indices int32, values fp32, params dict).  params not given are drawn from
+    the seed: poisson mean ~ U[4, 32]; power-law alpha ~ U[1.5, 3] (max length
+    min(n, 1e5)); band width ~ U{3..63} (offsets -w//2..w//2); block 8;
+    R-MAT (a, b, c) = (0.57, 0.19, 0.19) with n * U[4, 24] edges."""
+    import torch
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    p = dict(params or {})
+    if family == "poisson":
+        mean = p.setdefault("mean", float(rng.uniform(4, 32)))
+        lens = torch.poisson(torch.full((n,), mean, device=device), generator=gen).to(torch.int64)
+        lens = torch.clamp(lens, 0, n)
+        cols = _torch_sorted_random_cols(torch, lens, n, gen, device)
+    elif family == "powerlaw":
+        alpha = p.setdefault("alpha", float(rng.uniform(1.5, 3.0)))
+        max_len = min(n, 100_000)
+        u = torch.rand(n, generator=gen, device=device, dtype=torch.float64)
+        a = 1.0 - alpha
+        l = (1.0 + u * (max_len ** a - 1.0)) ** (1.0 / a)
+        lens = torch.clamp(torch.ceil(l), 1, max_len).to(torch.int64)
+        cols = _torch_sorted_random_cols(torch, lens, n, gen, device)
+    elif family == "banded":
+        w = p.setdefault("width", int(rng.integers(3, 64)))
+        h = w // 2
+        r = torch.arange(n, device=device)
+        lo = torch.clamp(r - h, min=0)
+        hi = torch.clamp(r + h, max=n - 1)
+        lens = hi - lo + 1
+        start = torch.zeros(n + 1, dtype=torch.int64, device=device)
+        torch.cumsum(lens, 0, out=start[1:])
+        rows = torch.repeat_interleave(r, lens)
+        cols = torch.arange(int(start[-1].item()), device=device) - start[rows] + lo[rows]
+    elif family == "blockdiag":
+        b = p.setdefault("block", 8)
+        r = torch.arange(n, device=device)
+        lo = (r // b) * b
+        lens = torch.clamp(lo + b, max=n) - lo
+        start = torch.zeros(n + 1, dtype=torch.int64, device=device)
+        torch.cumsum(lens, 0, out=start[1:])
+        rows = torch.repeat_interleave(r, lens)
+        cols = torch.arange(int(start[-1].item()), device=device) - start[rows] + lo[rows]
+    elif family == "rmat":
+        deg = p.setdefault("degree", float(rng.uniform(4, 24)))
+        m = int(n * deg)
+        k = max(1, int(np.ceil(np.log2(max(n, 2)))))
+        probs = torch.tensor([0.57, 0.19, 0.19, 0.05], device=device)
+        rr = torch.zeros(m, dtype=torch.int64, device=device)
+        cc = torch.zeros(m, dtype=torch.int64, device=device)
+        for bit in range(k):
+            q = torch.multinomial(probs, m, replacement=True, generator=gen)
+            rr |= ((q >> 1) & 1) << bit
+            cc |= (q & 1) << bit
+        key = torch.unique((rr % n) * n + (cc % n))
+        rows, cols = key // n, key % n
+        lens = torch.bincount(rows, minlength=n)
+    else:
+        raise ValueError(f"unknown family {family}")
+    indptr, cols, vals = _torch_rows_to_csr(torch, lens, cols, n, gen, device)
+    return indptr, cols, vals, p
+
+
+def selector_matrix_spec(seed):
+    """(family, n) of selector matrix `seed`: family by seed mod 5, n
+    log-uniform in [1e4, 2e6]."""
+    rng = np.random.Generator(np.random.PCG64(10_000_000 + seed))
+    family = SELECTOR_FAMILIES[seed % len(SELECTOR_FAMILIES)]
+    n = int(round(float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))))
+    return family, n

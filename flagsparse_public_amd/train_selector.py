@@ -1,0 +1,160 @@
+"""Collect autotuning timings, train the schedule selectors, evaluate them.
+
+    python -m flagsparse_public_amd.train_selector --out DIR [--n-train 2000]
+        [--n-test 200] [--max-n 2000000] [--resume]
+
+Matrices: seeds 5000.. (training) and 7000.. (held out), five families
+(``synthetic.SELECTOR_FAMILIES``) with n log-uniform in [1e4, 2e6], fp32
+values, generated on the GPU.  For every matrix all SpMV variants and all
+64-column SpMM variants are timed (``autotune``); the selectors are trained
+on the training rows with the library's HIP MLP kernels; on the held-out
+matrices the on-device selection is compared with exhaustive autotuning
+(regret) and with the fixed default schedule (speed-up).
+
+Writes DIR/{spmv,spmm64}_db.npz, DIR/{spmv,spmm64}_f32_mlp.json and
+DIR/eval.json; prints one progress line per 20 matrices.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, synthetic
+from .autotune import (SPMM_DEFAULT_INDEX, SPMM_VARIANTS, SPMV_DEFAULT_INDEX, SPMV_VARIANTS,
+                       TimingDB, tune_spmm, tune_spmv)
+from .selector import Selector
+from .sparse_operations.spmm_csr import prepare_spmm_csr_opt
+from .sparse_operations.spmv_csr import prepare_spmv_csr
+
+TRAIN_SEED0 = 5000
+TEST_SEED0 = 7000
+
+
+def _log(msg):
+    print(f"[train_selector] {msg}", flush=True)
+
+
+def collect(seeds, device, dbs, max_n, reps, t0, label):
+    dense_cache = {}
+    for k, seed in enumerate(seeds):
+        family, n = synthetic.selector_matrix_spec(seed)
+        n = min(n, max_n)
+        indptr, cols, vals, params = synthetic.family_csr_torch(family, n, seed, device)
+        nnz = int(vals.numel())
+        pv = prepare_spmv_csr(vals, cols, indptr, (n, n))
+        stats = _lib.analyze_device(pv.csr_struct(), device)
+        feats = _lib.features(stats, n, n, nnz, device).double().cpu().numpy()
+        g = torch.Generator(device=device)
+        g.manual_seed(seed + 1)
+        x = torch.rand(n, generator=g, device=device) * 2 - 1
+        tv = tune_spmv(pv, x, reps=reps)
+        dbs["spmv"].add(seed, family, feats, tv, n, nnz)
+        if n not in dense_cache:
+            dense_cache.clear()
+            dense_cache[n] = torch.rand((n, 64), generator=g, device=device) * 2 - 1
+        pm = prepare_spmm_csr_opt(vals, cols, indptr, (n, n))
+        tm = tune_spmm(pm, dense_cache[n], reps=max(3, reps - 2))
+        dbs["spmm64"].add(seed, family, feats, tm, n, nnz)
+        del pv, pm, indptr, cols, vals, x
+        if (k + 1) % 20 == 0 or k + 1 == len(seeds):
+            _log(f"{label} {k + 1}/{len(seeds)} matrices, {time.time() - t0:.0f}s "
+                 f"(last: {family} n={n} nnz={nnz}, spmv best {tv.min():.1f}us, "
+                 f"spmm best {tm.min():.1f}us)")
+
+
+def evaluate(sel: Selector, db: TimingDB, default_index: int, device):
+    _, fams, feats, times, _ = db.arrays()
+    choices = []
+    for f in feats:
+        fd = torch.as_tensor(f.astype(np.float32), device=device)
+        choices.append(int(sel.select_on_device(fd).item()))
+    choices = np.asarray(choices)
+    rows = np.arange(len(choices))
+    t_sel = times[rows, choices]
+    t_best = times.min(axis=1)
+    t_def = times[:, default_index]
+    regret = t_sel / t_best
+    speedup = t_def / t_sel
+    out = {
+        "n": int(len(choices)),
+        "geomean_regret": float(np.exp(np.mean(np.log(regret)))),
+        "frac_within_5pct_of_exhaustive": float(np.mean(regret <= 1.05)),
+        "total_time_selected_over_exhaustive": float(t_sel.sum() / t_best.sum()),
+        "geomean_speedup_vs_default": float(np.exp(np.mean(np.log(speedup)))),
+        "total_time_default_over_selected": float(t_def.sum() / t_sel.sum()),
+        "exhaustive_geomean_speedup_vs_default": float(np.exp(np.mean(np.log(t_def / t_best)))),
+        "per_family": {},
+    }
+    for fam in sorted(set(fams.tolist())):
+        m = fams == fam
+        out["per_family"][fam] = {
+            "geomean_regret": float(np.exp(np.mean(np.log(regret[m])))),
+            "geomean_speedup_vs_default": float(np.exp(np.mean(np.log(speedup[m])))),
+        }
+    return out, choices
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--n-train", type=int, default=2000)
+    ap.add_argument("--n-test", type=int, default=200)
+    ap.add_argument("--max-n", type=int, default=2_000_000)
+    ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--resume", action="store_true", help="reuse DIR/*_db.npz if present")
+    ap.add_argument("--collect-only", action="store_true")
+    args = ap.parse_args(argv)
+    os.makedirs(args.out, exist_ok=True)
+    device = torch.device("cuda", 0)
+    name = torch.cuda.get_device_name(device)
+    t0 = time.time()
+    paths = {s: {op: os.path.join(args.out, f"{op}_{s}_db.npz") for op in ("spmv", "spmm64")}
+             for s in ("train", "test")}
+    dbs = {}
+    for split, seed0, count in (("train", TRAIN_SEED0, args.n_train), ("test", TEST_SEED0, args.n_test)):
+        have = {}
+        if args.resume and all(os.path.exists(p) for p in paths[split].values()):
+            have = {op: TimingDB.load(p) for op, p in paths[split].items()}
+        if not have:
+            have = {"spmv": TimingDB("spmv", len(SPMV_VARIANTS), name),
+                    "spmm64": TimingDB("spmm64", len(SPMM_VARIANTS), name)}
+        done = set(have["spmv"].ids)
+        seeds = [s for s in range(seed0, seed0 + count) if s not in done]
+        if seeds:
+            collect(seeds, device, have, args.max_n, args.reps, t0, split)
+            for op, db in have.items():
+                db.save(paths[split][op])
+        dbs[split] = have
+    if args.collect_only:
+        return 0
+    report = {"device": name, "n_train": len(dbs["train"]["spmv"]),
+              "n_test": len(dbs["test"]["spmv"])}
+    for op, default_index in (("spmv", SPMV_DEFAULT_INDEX), ("spmm64", SPMM_DEFAULT_INDEX)):
+        _, _, Xtr, Ttr, _ = dbs["train"][op].arrays()
+        sel = Selector.train(op, Xtr, Ttr, device, epochs=args.epochs,
+                             log=lambda m, op=op: _log(f"{op}: {m}"))
+        sel.meta.update({"device": name, "default_index": default_index})
+        ev, _ = evaluate(sel, dbs["test"][op], default_index, device)
+        tr, _ = evaluate(sel, dbs["train"][op], default_index, device)
+        report[op] = {"held_out": ev, "train": {k: tr[k] for k in
+                                                ("geomean_regret", "geomean_speedup_vs_default")}}
+        sel.meta["held_out"] = ev
+        sel.save(os.path.join(args.out, f"{op}_f32_mlp.json"))
+        _log(f"{op}: held-out regret {ev['geomean_regret']:.3f}, "
+             f"speed-up vs default {ev['geomean_speedup_vs_default']:.3f}")
+    with open(os.path.join(args.out, "eval.json"), "w") as fh:
+        json.dump(report, fh, indent=1)
+    print(json.dumps(report), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -149,6 +149,131 @@ int fs_spmm_csr(const fs_csr *A, const void *B, int64_t ldb, void *C, int64_t ld
 #define FS_STATS_LEN 16
 int fs_csr_analyze(const fs_csr *A, int64_t *stats, void *stream);
 
+/* ------------------------------------------- split-precision value storage */
+/*
+ * Storage formats for the values of an fp32 matrix (x, y stay fp32; the sum is
+ * fp32 with compensated summation in the chunked long-row role).  No
+ * counterpart in the reference, whose values are always stored in the call's
+ * dtype; used by callers that trade value precision for bytes.
+ */
+enum {
+  FS_STORE_BF16X2 = 1,       /* hi = bf16(v), lo = bf16(v - hi); a = hi + lo  */
+  FS_STORE_BF16 = 2,         /* hi plane only                                 */
+  FS_STORE_F16_ROWSCALE = 3  /* h = fp16(v / s_r), s_r = 2^ceil(log2 max|row|) */
+};
+
+/*
+ * Encode A's fp32 values (A->values) into a storage format: hi / lo are nnz
+ * uint16 planes (lo only for FS_STORE_BF16X2; FS_STORE_F16_ROWSCALE writes the
+ * fp16 plane to hi), row_scale has n_rows floats (FS_STORE_F16_ROWSCALE only).
+ */
+int fs_split_values(int32_t format, const fs_csr *A, void *hi, void *lo, float *row_scale,
+                    void *stream);
+
+/*
+ * y = alpha * A x + beta * y with A's values taken from a storage format
+ * written by fs_split_values (A->values is not read).  A must be FS_F32 with
+ * FS_I32 column indices.
+ */
+int fs_spmv_csr_split(const fs_csr *A, int32_t format, const void *hi, const void *lo,
+                      const float *row_scale, const void *x, void *y, double alpha, double beta,
+                      const fs_plan *plan, void *stream);
+
+/* ------------------------------------------------- batched persistent path */
+/*
+ * Many independent fp32 / int32-column CSR problems in one persistent launch
+ * (replayable from a captured graph).  No counterpart in the reference, which
+ * launches per matrix and row bucket; used by callers with many small or
+ * uneven matrices.
+ */
+enum { FS_BATCH_SPMV = 0, FS_BATCH_SPMM = 1 };
+
+typedef struct fs_batch_entry {
+  fs_csr A;              /* FS_F32 values, FS_I32 columns                     */
+  const void *x;         /* SpMV operand (n_cols), or NULL                    */
+  void *y;               /* SpMV result (n_rows)                              */
+  const void *B;         /* SpMM operand (n_cols x n_dense, ldb), or NULL     */
+  void *C;               /* SpMM result (n_rows x n_dense, ldc)               */
+  int64_t n_dense, ldb, ldc;
+  int32_t group_lanes;   /* SpMV lanes per row: 1..64, power of two           */
+  int32_t reserved;
+} fs_batch_entry;
+
+/* One unit of work: rows [row_begin, row_end) of matrix `matrix`. */
+typedef struct fs_batch_item {
+  int32_t matrix;
+  int32_t kind;          /* FS_BATCH_SPMV / FS_BATCH_SPMM */
+  int64_t row_begin;
+  int64_t row_end;
+} fs_batch_item;
+
+/* Bytes of one packed matrix record; fs_batch_pack writes n_entries records
+ * into host memory, which the caller copies to the device. */
+int64_t fs_batch_matrix_bytes(void);
+int fs_batch_pack(const fs_batch_entry *entries, int64_t n_entries, void *packed_host);
+
+/*
+ * Run every item: y = alpha A x + beta y for SpMV items, C = alpha A B + beta C
+ * for SpMM items.  packed_dev: the packed records in device memory; items:
+ * device array; queue: two int32 in device memory, zero before the first call
+ * and zero again after every call.  grid: persistent block count (e.g. a
+ * small multiple of the CU count).  Single-row SpMV items longer than
+ * long_row entries are reduced by a whole block.  Calls sharing a queue must
+ * be ordered on one stream.
+ */
+int fs_batch_run(const void *packed_dev, int64_t n_matrices, const fs_batch_item *items,
+                 int64_t n_items, int64_t long_row, int32_t *queue, int32_t grid, double alpha,
+                 double beta, void *stream);
+
+/* --------------------------------------------------- learned kernel choice */
+/*
+ * A 16 -> hidden -> hidden -> out_dim perceptron (ReLU hidden layers) whose
+ * outputs are predicted relative times of out_dim schedule variants; the
+ * selector picks the smallest.  params is one fp32 array laid out as
+ * W1[in][hidden], b1[hidden], W2[hidden][hidden], b2[hidden],
+ * W3[hidden][out], b3[out] (row-major).  No counterpart in the reference,
+ * which picks launch parameters from static tables (spmv_csr.py:128-153,
+ * spmm_csr_opt_alg2.py:175-304).
+ */
+typedef struct fs_mlp {
+  int32_t in_dim;
+  int32_t hidden;
+  int32_t out_dim;
+  int32_t reserved;
+  float *params;
+} fs_mlp;
+
+int64_t fs_mlp_param_count(int32_t in_dim, int32_t hidden, int32_t out_dim);
+
+/* 16 float features of a matrix from the FS_STATS_LEN statistics of
+ * fs_csr_analyze (device pointers): log2(n_rows+1), log2(n_cols+1),
+ * log2(nnz+1), mean, std, std/mean, log2(max+1), min, empty fraction, six
+ * length-bin fractions, fraction of entries in rows longer than 256. */
+int fs_csr_features(const int64_t *stats, int64_t n_rows, int64_t n_cols, int64_t nnz,
+                    float *features, void *stream);
+
+/* Forward pass over batch samples X[batch][in]; H1, H2 ([batch][hidden]) may be
+ * null when no backward pass follows; Y is [batch][out]. */
+int fs_mlp_forward(const fs_mlp *net, const float *X, int64_t batch, float *H1, float *H2,
+                   float *Y, void *stream);
+
+/* Gradient (same layout as params) of loss = sum(0.5 (M (Y - T))^2) /
+ * (batch * out) and the loss itself (one float, may be null).  workspace of
+ * fs_mlp_workspace_bytes bytes.  Deterministic. */
+int64_t fs_mlp_workspace_bytes(int32_t in_dim, int32_t hidden, int32_t out_dim, int64_t batch);
+int fs_mlp_backward(const fs_mlp *net, const float *X, const float *H1, const float *H2,
+                    const float *Y, const float *T, const float *M, int64_t batch, float *grad,
+                    float *loss, void *workspace, int64_t workspace_bytes, void *stream);
+
+/* Bias-corrected Adam step (step counts from 1) over n parameters. */
+int fs_adam_step(float *params, const float *grad, float *m, float *v, int64_t n, double lr,
+                 double beta1, double beta2, double eps, int64_t step, void *stream);
+
+/* Standardise features ((f - mu) / sd), forward them and write the index of the
+ * smallest output to *choice (and all outputs to scores unless null). */
+int fs_mlp_select(const fs_mlp *net, const float *features, const float *mu, const float *sd,
+                  int32_t *choice, float *scores, void *stream);
+
 /* Message of the calling thread's last failed call ("" if none). */
 const char *fs_last_error(void);
 

@@ -32,8 +32,10 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header(tmp_path):
     """ctypes mirrors of fs_csr / fs_plan have the C compiler's offsets."""
     import subprocess
-    fields = {"fs_csr": [f for f, _ in _lib.FsCsr._fields_],
-              "fs_plan": [f for f, _ in _lib.FsPlan._fields_]}
+    classes = {"fs_csr": _lib.FsCsr, "fs_plan": _lib.FsPlan, "fs_batch_entry": _lib.FsBatchEntry,
+               "fs_mlp": _lib.FsMlp}
+    fields = {st: [f for f, _ in cls._fields_] for st, cls in classes.items()}
+    fields["fs_batch_item"] = ["matrix", "kind", "row_begin", "row_end"]
     lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():
         lines.append(f'printf("{st} %zu\\n", sizeof({st}));')
@@ -46,10 +48,13 @@ def test_struct_layouts_match_header(tmp_path):
     subprocess.run(["gcc", "-std=c11", str(src), "-o", str(exe)], check=True)
     out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
                                                  check=True).stdout.splitlines())
-    for st, cls in (("fs_csr", _lib.FsCsr), ("fs_plan", _lib.FsPlan)):
+    for st, cls in classes.items():
         assert int(out[st]) == ctypes.sizeof(cls)
         for f in fields[st]:
             assert int(out[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
+    # work items are built as int64 triples: (matrix | kind << 32, row_begin, row_end)
+    assert [int(out[f"fs_batch_item.{f}"]) for f in fields["fs_batch_item"]] == [0, 4, 8, 16]
+    assert int(out["fs_batch_item"]) == 24
 
 
 def test_version_and_partials_bytes():

@@ -253,3 +253,41 @@ def test_powerlaw_lengths_distribution():
     assert l.min() >= 1 and l.max() <= 100000
     # P(len > 128) = (1/128 - 1/1e5) / (1 - 1/1e5) for the ceiling of l > 128
     assert abs((l > 128).mean() - (1 / 128 - 1e-5)) < 1e-3
+
+
+def test_batched_cuts_and_items():
+    from flagsparse_public_amd.batched import BatchedCsr, _cuts
+    lens = torch.tensor([3, 0, 5000, 2, 2, 9000, 1, 0, 4])
+    ip = torch.zeros(lens.numel() + 1, dtype=torch.int64)
+    ip[1:] = torch.cumsum(lens, 0)
+    cuts = _cuts(ip, 1.0, 100, 3, long_row=4096)
+    c = cuts.tolist()
+    assert c[0] == 0 and c[-1] == lens.numel() and c == sorted(set(c))
+    # both long rows (2 and 5) are isolated into single-row items
+    for r in (2, 5):
+        assert r in c and r + 1 in c
+    # no item spans more than 3 rows
+    assert max(b - a for a, b in zip(c[:-1], c[1:])) <= 3
+    items = BatchedCsr._items(7, 1, cuts)
+    assert items.shape[1] == 3 and (items[:, 0] == (7 | (1 << 32))).all()
+    assert items[:, 1].tolist() == c[:-1] and items[:, 2].tolist() == c[1:]
+
+
+def test_batch_pack_validates_on_host():
+    import ctypes
+    from flagsparse_public_amd import _lib
+    from flagsparse_public_amd.batched import FsBatchEntry, _lib_batch
+    lib = _lib_batch()
+    assert lib.fs_batch_matrix_bytes() >= 8 * 11
+    ents = (FsBatchEntry * 2)()
+    for e in ents:
+        e.A = _lib.FsCsr(value_type=_lib.FS_F32, index_type=_lib.FS_I32, ptr_type=_lib.FS_I32,
+                         reserved=0, n_rows=4, n_cols=4, nnz=4, row_ptr=64, col_idx=128, values=256)
+        e.n_dense, e.ldb, e.ldc, e.group_lanes = 8, 8, 8, 4
+    buf = (ctypes.c_uint8 * (lib.fs_batch_matrix_bytes() * 2))()
+    assert lib.fs_batch_pack(ents, 2, ctypes.addressof(buf)) == _lib.FS_OK
+    ents[1].group_lanes = 3
+    assert lib.fs_batch_pack(ents, 2, ctypes.addressof(buf)) == _lib.FS_ERR_VALUE
+    ents[1].group_lanes = 4
+    ents[1].A.value_type = _lib.FS_F64
+    assert lib.fs_batch_pack(ents, 2, ctypes.addressof(buf)) == _lib.FS_ERR_TYPE
