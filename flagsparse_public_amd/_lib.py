@@ -71,9 +71,11 @@ class FsPlan(ctypes.Structure):
         ("long_chunk_ptr", ctypes.c_void_p),
         ("n_chunks", ctypes.c_int64),
         ("chunk_long", ctypes.c_void_p),
-        ("counters", ctypes.c_void_p),
         ("partials", ctypes.c_void_p),
         ("partials_bytes", ctypes.c_int64),
+        ("tile_nnz", ctypes.c_int64),
+        ("n_tiles", ctypes.c_int64),
+        ("tile_start", ctypes.c_void_p),
     ]
 
 

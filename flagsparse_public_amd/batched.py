@@ -46,9 +46,11 @@ def _cuts(indptr, weight_per_entry, target, max_rows, long_row=None):
     dev = indptr.device
     nnz = int(indptr[-1].item())
     step = max(1, int(target // max(weight_per_entry, 1e-9)))
-    marks = torch.arange(step, nnz, step, device=dev, dtype=torch.int64)
+    marks = (torch.arange(step, nnz, step, device=dev, dtype=torch.int64) if nnz > step
+             else torch.zeros(0, dtype=torch.int64, device=dev))
     by_nnz = torch.searchsorted(indptr[1:].to(torch.int64), marks, right=True)
-    by_rows = torch.arange(max_rows, n, max_rows, device=dev, dtype=torch.int64)
+    by_rows = (torch.arange(max_rows, n, max_rows, device=dev, dtype=torch.int64) if n > max_rows
+               else torch.zeros(0, dtype=torch.int64, device=dev))
     parts = [torch.zeros(1, dtype=torch.int64, device=dev), by_nnz, by_rows,
              torch.full((1,), n, dtype=torch.int64, device=dev)]
     if long_row is not None and n > 0:

@@ -90,7 +90,18 @@ __global__ __launch_bounds__(kAnBlock) void analyze_pass(const P* __restrict__ p
     if (len > 256) longnnz += len;
   }
   int64_t cmin = INT64_MAX, cmax = -1;
-  for (int64_t j = t0; j < nnz; j += stride) {
+  int64_t j = t0;
+  for (; j + 3 * stride < nnz; j += 4 * stride) {  // four independent loads in flight
+    int64_t c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = (int64_t)idx[j + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      cmin = c[u] < cmin ? c[u] : cmin;
+      cmax = c[u] > cmax ? c[u] : cmax;
+    }
+  }
+  for (; j < nnz; j += stride) {
     const int64_t c = (int64_t)idx[j];
     cmin = c < cmin ? c : cmin;
     cmax = c > cmax ? c : cmax;
@@ -115,8 +126,10 @@ int launch_analyze(const fs_csr* A, int64_t* st, hipStream_t s) {
   if (rc) return rc;
   const int64_t work = A->n_rows > A->nnz ? A->n_rows : A->nnz;
   if (work == 0) return FS_OK;
+  // few blocks: each reduces in LDS first, so every statistic sees at most
+  // `blocks` same-address atomics (they serialise)
   int64_t blocks = (work + kAnBlock * 8 - 1) / (kAnBlock * 8);
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > 512) blocks = 512;
   analyze_pass<I, P><<<(unsigned)blocks, kAnBlock, 0, s>>>(ptr, idx, A->n_rows, A->nnz, st);
   return check_launch("fs_csr_analyze");
 }

@@ -101,8 +101,16 @@ int launch_split_s(const fs_csr* M, const VS& vs, const void* x, void* y, double
   if (grid == 0) return FS_OK;
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr_split: grid of %lld blocks too large", (long long)grid);
-  spmv_csr_kernel<Plain<float>, S, 4><<<dim3((unsigned)grid), dim3(kBlock), 0, st>>>(p);
-  return check_launch("fs_spmv_csr_split");
+  if (pl->tile_nnz > 0) {
+    if (tile_lds_bytes<double>(pl->tile_nnz) > 65536)
+      return fail(FS_ERR_VALUE, "fs_spmv_csr_split: tile_nnz %lld needs more than 64 KB of LDS",
+                  (long long)pl->tile_nnz);
+    spmv_launch_tiles<Plain<float>, 4>(p, grid, st);
+  } else {
+    spmv_csr_kernel<Plain<float>, S, 4><<<dim3((unsigned)grid), dim3(kBlock), 0, st>>>(p);
+  }
+  const int rc = check_launch("fs_spmv_csr_split");
+  return rc ? rc : spmv_launch_fixup<Plain<float>>(p, pl, st, "fs_spmv_csr_split");
 }
 
 template <typename VS, typename P>
@@ -130,7 +138,10 @@ int launch_split_ptr(const fs_csr* M, const VS& vs, const void* x, void* y, doub
 }
 
 static int check_split_csr(const fs_csr* A, const char* who) {
-  int rc = check_csr(A, who);
+  // the values live in the split planes: check the descriptor as if A->values were set
+  fs_csr probe = *A;
+  if (!probe.values) probe.values = probe.col_idx;
+  int rc = check_csr(&probe, who);
   if (rc) return rc;
   if (A->value_type != FS_F32) return fail(FS_ERR_TYPE, "%s: split storage is for fp32 matrices", who);
   if (A->index_type != FS_I32) return fail(FS_ERR_TYPE, "%s: split storage needs int32 column indices", who);

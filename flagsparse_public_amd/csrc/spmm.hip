@@ -6,8 +6,9 @@
 // group reads one contiguous B-row segment per entry.  U entries are kept in
 // flight per lane.  Rows longer than short_max are cut into chunks reduced by
 // whole wavefronts (64/G groups interleave over the chunk's entries and are
-// combined with xor exchanges), with the same last-arriver fix-up as SpMV:
-// the row's partials are summed in chunk order.
+// combined with xor exchanges); a row spanning several chunks stores one
+// partial row per chunk, and spmm_fixup_kernel (a second, small launch) sums
+// them in chunk order.
 //
 // Replaces the reference's ALG1 row x column-tile kernel (spmm_csr.py:92-134,
 // 525-599) and the bucketed opt / alg2 kernels (spmm_csr.py:602-919,
@@ -39,7 +40,6 @@ struct SpmmParams {
   const int64_t* long_rows;
   const int64_t* long_chunk_ptr;
   const int32_t* chunk_long;
-  int32_t* counters;  // [n_long * n_tiles]
   void* partials;     // [n_chunks * n_dense] accumulators
   int xcd_remap;
 };
@@ -96,23 +96,6 @@ __device__ __forceinline__ void load_vec(C (&out)[VW], const S* __restrict__ p, 
 #pragma unroll
     for (int k = 0; k < VW; ++k) out[k] = load_as<C>(p, off + k);
   }
-}
-
-template <typename A> __device__ __forceinline__ void store_partial_mm(A* p, const A& a) {
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(&a);
-  uint32_t* dst = reinterpret_cast<uint32_t*>(p);
-#pragma unroll
-  for (int w = 0; w < (int)(sizeof(A) / 4); ++w)
-    __hip_atomic_store(dst + w, src[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename A> __device__ __forceinline__ A load_partial_mm(const A* p) {
-  A a;
-  uint32_t* dst = reinterpret_cast<uint32_t*>(&a);
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
-#pragma unroll
-  for (int w = 0; w < (int)(sizeof(A) / 4); ++w)
-    dst[w] = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return a;
 }
 
 // Accumulate entries j = s + first, s + first + step, ... < e into acc[VW] for
@@ -180,34 +163,31 @@ __device__ __forceinline__ void spmm_chunk_role(const SpmmParams<V, I, P>& p, in
       for (int k = 0; k < VW; ++k) write_out(p.C, row * p.ldc + n0 + k, acc[k].value(), p.alpha, p.beta);
     return;
   }
-  CA* parts = reinterpret_cast<CA*>(p.partials);
+  CA* parts = reinterpret_cast<CA*>(p.partials);  // summed by spmm_fixup_kernel
   if (grp == 0 && colok)
 #pragma unroll
-    for (int k = 0; k < VW; ++k) store_partial_mm(parts + c * p.n_dense + n0 + k, acc[k]);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  int last = 0;
-  if (lane == 0) {
-    const int old = __hip_atomic_fetch_add(p.counters + li * p.n_tiles + tile, 1, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-    last = (old == (int)nc - 1);
+    for (int k = 0; k < VW; ++k) parts[c * p.n_dense + n0 + k] = acc[k];
+}
+
+// Second pass for rows cut into several chunks: one wavefront per long row;
+// lane n sums column n's chunk partials in chunk order (then n + 64, ...).
+template <typename A, typename V, typename I, typename P>
+__global__ __launch_bounds__(kMmBlock) void spmm_fixup_kernel(SpmmParams<V, I, P> p, int64_t n_long) {
+  using CA = typename MmChunkAcc<A>::T;
+  const int lane = threadIdx.x % kWave;
+  const int64_t li = (int64_t)blockIdx.x * kMmWaves + threadIdx.x / kWave;
+  if (li >= n_long) return;
+  const int64_t c0 = p.long_chunk_ptr[li];
+  const int64_t nc = p.long_chunk_ptr[li + 1] - c0;
+  if (nc <= 1) return;
+  const int64_t row = p.long_rows[li];
+  const CA* parts = reinterpret_cast<const CA*>(p.partials);
+  for (int64_t n = lane; n < p.n_dense; n += kWave) {
+    CA tot;
+    tot.init();
+    for (int64_t k = 0; k < nc; ++k) tot.merge(parts[(c0 + k) * p.n_dense + n]);
+    write_out(p.C, row * p.ldc + n, tot.value(), p.alpha, p.beta);
   }
-  last = __builtin_amdgcn_readfirstlane(last);
-  if (!last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  CA tot[VW];
-#pragma unroll
-  for (int k = 0; k < VW; ++k) tot[k].init();
-  if (colok)
-    for (int64_t q = grp; q < nc; q += kGroups)
-#pragma unroll
-      for (int k = 0; k < VW; ++k) tot[k].merge(load_partial_mm(parts + (c0 + q) * p.n_dense + n0 + k));
-#pragma unroll
-  for (int k = 0; k < VW; ++k) cross_group_reduce<G>(tot[k]);
-  if (grp == 0 && colok)
-#pragma unroll
-    for (int k = 0; k < VW; ++k) write_out(p.C, row * p.ldc + n0 + k, tot[k].value(), p.alpha, p.beta);
-  if (lane == 0)
-    __hip_atomic_store(p.counters + li * p.n_tiles + tile, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename V, typename A, typename I, typename P, int G, int VW, int U>
@@ -262,7 +242,6 @@ int launch_spmm_gv(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t
   p.long_rows = pl->long_rows;
   p.long_chunk_ptr = pl->long_chunk_ptr;
   p.chunk_long = pl->chunk_long;
-  p.counters = pl->counters;
   p.partials = pl->partials;
   p.xcd_remap = pl->xcd_remap;
   const int64_t rows_per_block = kMmBlock / G;
@@ -272,6 +251,10 @@ int launch_spmm_gv(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmm_csr: grid of %lld blocks too large", (long long)grid);
   spmm_csr_kernel<V, A, I, P, G, VW, 4><<<dim3((unsigned)grid), dim3(kMmBlock), 0, st>>>(p);
+  int rc = check_launch("fs_spmm_csr");
+  if (rc || pl->n_chunks <= pl->n_long) return rc;  // no row spans several chunks
+  const int64_t blocks = (pl->n_long + kMmWaves - 1) / kMmWaves;
+  spmm_fixup_kernel<A><<<dim3((unsigned)blocks), dim3(kMmBlock), 0, st>>>(p, pl->n_long);
   return check_launch("fs_spmm_csr");
 }
 

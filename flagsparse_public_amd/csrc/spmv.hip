@@ -33,6 +33,23 @@ int launch_spmv_s(const fs_csr* M, const void* x, void* y, double alpha, double 
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr: grid of %lld blocks too large", (long long)grid);
   const dim3 g((unsigned)grid), b(kBlock);
+  if (pl->tile_nnz > 0) {
+    using Sum = typename TileSum<typename A::compute_t>::T;
+    if (tile_lds_bytes<Sum>(pl->tile_nnz) > 65536)
+      return fail(FS_ERR_VALUE, "fs_spmv_csr: tile_nnz %lld needs more than 64 KB of LDS",
+                  (long long)pl->tile_nnz);
+    if constexpr (kHotSpmv<V, I>) {
+      switch (pl->unroll) {
+        case 2: spmv_launch_tiles<A, 2>(p, grid, st); break;
+        case 8: spmv_launch_tiles<A, 8>(p, grid, st); break;
+        default: spmv_launch_tiles<A, 4>(p, grid, st); break;
+      }
+    } else {
+      spmv_launch_tiles<A, 4>(p, grid, st);
+    }
+    const int rc = check_launch("fs_spmv_csr");
+    return rc ? rc : spmv_launch_fixup<A>(p, pl, st, "fs_spmv_csr");
+  }
   if constexpr (kHotSpmv<V, I>) {
     switch (pl->unroll) {
       case 2: spmv_csr_kernel<A, S, 2><<<g, b, 0, st>>>(p); break;
@@ -42,7 +59,8 @@ int launch_spmv_s(const fs_csr* M, const void* x, void* y, double alpha, double 
   } else {
     spmv_csr_kernel<A, S, 4><<<g, b, 0, st>>>(p);
   }
-  return check_launch("fs_spmv_csr");
+  const int rc = check_launch("fs_spmv_csr");
+  return rc ? rc : spmv_launch_fixup<A>(p, pl, st, "fs_spmv_csr");
 }
 
 template <typename V, typename A, typename I, typename P>

@@ -52,7 +52,7 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
   if (P->short_max < 0) return fail(FS_ERR_VALUE, "%s: short_max must be >= 0", who);
   if (P->n_long < 0 || P->n_chunks < 0) return fail(FS_ERR_VALUE, "%s: negative long-row counts", who);
   if (P->n_long > 0) {
-    if (!P->long_rows || !P->long_chunk_ptr || !P->chunk_long || !P->counters)
+    if (!P->long_rows || !P->long_chunk_ptr || !P->chunk_long)
       return fail(FS_ERR_VALUE, "%s: long-row metadata pointers are null", who);
     if (P->chunk_nnz <= 0 || P->chunk_nnz % kWave != 0)
       return fail(FS_ERR_VALUE, "%s: chunk_nnz must be a positive multiple of 64", who);
@@ -65,6 +65,14 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
                   (long long)P->partials_bytes, (long long)need);
   } else if (P->n_chunks != 0) {
     return fail(FS_ERR_VALUE, "%s: chunks listed without long rows", who);
+  }
+  if (P->tile_nnz < 0 || P->n_tiles < 0) return fail(FS_ERR_VALUE, "%s: negative tile fields", who);
+  if (P->tile_nnz > 0) {
+    if (n_dense != 1) return fail(FS_ERR_VALUE, "%s: the tile role is for SpMV", who);
+    if (P->short_max > P->tile_nnz)
+      return fail(FS_ERR_VALUE, "%s: tile role needs short_max <= tile_nnz", who);
+    if (A->n_rows > 0 && (P->n_tiles == 0 || !P->tile_start))
+      return fail(FS_ERR_VALUE, "%s: tile role needs tile_start", who);
   }
   return FS_OK;
 }

@@ -4,8 +4,8 @@ A schedule sends every row of length <= ``short_max`` to the short-row role
 (groups of ``group_lanes`` lanes, natural row order) and cuts every longer row
 into chunks of ``chunk_nnz`` entries reduced by one wavefront each.  The plan
 holds, in device memory: the long-row ids, each long row's first chunk, the
-long-row ordinal of every chunk, the per-row arrival counters (zero between
-calls) and the chunk-partial workspace.
+long-row ordinal of every chunk and the chunk-partial workspace (read by the
+second, fix-up launch for rows that span several chunks).
 
 This replaces the reference's row-length buckets (spmv_csr.py:128-153,
 345-391; spmm_csr.py:395-477; spmm_csr_opt_alg2.py:60-166, 307-335): the
@@ -30,6 +30,7 @@ class Schedule:
     unroll: int = 4           # entries in flight per lane
     xcd_remap: int = 0        # 1: contiguous row ranges per XCD
     col_tile: int = 0         # SpMM dense columns per lane group (0: all)
+    tile_nnz: int = 0         # SpMV: > 0 selects the tile role for short rows
 
     def key(self) -> tuple:
         return dataclasses.astuple(self)
@@ -42,12 +43,49 @@ def _pow2_ceil(v: float) -> int:
     return p
 
 
+TILE_ROWS = 512  # rows per tile at most (the kernel's LDS row-pointer slots)
+
+
 def default_spmv_schedule(n_rows: int, nnz: int) -> Schedule:
-    """Fixed rule used when no selector is supplied: group width near half the
-    mean row length, rows over 16 entries per lane chunked in 512-entry chunks."""
+    """Fixed rule used when no selector is supplied: tiles of ~2048 entries for
+    rows up to 2048 entries, longer rows in 4096-entry chunks."""
+    return Schedule(group_lanes=8, short_max=2048, chunk_nnz=4096, unroll=8, tile_nnz=2048)
+
+
+def rowgroup_spmv_schedule(n_rows: int, nnz: int) -> Schedule:
+    """The lane-group-per-row schedule: group width near half the mean row
+    length, rows over 16 entries per lane chunked in 512-entry chunks."""
     mean = nnz / n_rows if n_rows > 0 else 0.0
     s = _pow2_ceil(max(mean / 2.0, 1.0))
     return Schedule(group_lanes=s, short_max=max(16 * s, 64), chunk_nnz=512, unroll=4)
+
+
+def tile_starts(row_lengths: torch.Tensor, short_max: int, tile_nnz: int,
+                max_rows: int = TILE_ROWS) -> torch.Tensor:
+    """First row of every tile, plus n_rows at the end (int64, on the device).
+
+    Cuts: where the running count of short-row entries crosses a multiple of
+    tile_nnz, every max_rows rows, and around every row longer than
+    short_max (which becomes a one-row tile that the tile role skips).  A
+    tile then holds < tile_nnz + short_max entries and <= max_rows rows.
+    """
+    dev = row_lengths.device
+    n = row_lengths.numel()
+    if n == 0:
+        return torch.zeros(1, dtype=torch.int64, device=dev)
+    lens = row_lengths.to(torch.int64)
+    long_mask = lens > short_max
+    eff = torch.where(long_mask, torch.zeros_like(lens), lens)
+    cum_excl = torch.cumsum(eff, 0) - eff
+    key = cum_excl // int(tile_nnz)
+    parts = [torch.zeros(1, dtype=torch.int64, device=dev),
+             torch.nonzero(key[1:] != key[:-1], as_tuple=False).flatten() + 1,
+             torch.arange(0, n, max_rows, dtype=torch.int64, device=dev),
+             torch.full((1,), n, dtype=torch.int64, device=dev)]
+    longs = torch.nonzero(long_mask, as_tuple=False).flatten()
+    if longs.numel():
+        parts += [longs, longs + 1]
+    return torch.unique(torch.cat(parts))
 
 
 def default_spmm_schedule(n_rows: int, nnz: int, n_dense: int) -> Schedule:
@@ -57,8 +95,8 @@ def default_spmm_schedule(n_rows: int, nnz: int, n_dense: int) -> Schedule:
 class RowSplitPlan:
     """Device metadata of one schedule for one matrix (see module docstring).
 
-    Calls that share a plan must be ordered on one stream: the arrival
-    counters and partials are reused by every call.
+    Calls that share a plan must be ordered on one stream: the chunk
+    partials are reused by every call.
     """
 
     def __init__(self, indptr: torch.Tensor, n_rows: int, value_dtype, schedule: Schedule,
@@ -86,7 +124,15 @@ class RowSplitPlan:
             self.long_chunk_ptr = torch.zeros(1, dtype=torch.int64, device=device)
             self.n_chunks = 0
             self.chunk_long = torch.zeros(0, dtype=torch.int32, device=device)
-        self._work = {}  # n_dense -> (counters, partials)
+        if schedule.tile_nnz > 0:
+            if schedule.short_max > schedule.tile_nnz:
+                raise ValueError("tile role needs short_max <= tile_nnz")
+            self.tile_start = tile_starts(row_lengths, schedule.short_max, schedule.tile_nnz)
+            self.n_tiles = int(self.tile_start.numel()) - 1
+        else:
+            self.tile_start = None
+            self.n_tiles = 0
+        self._work = {}  # n_dense -> partials
         self._structs = {}
 
     def struct(self, n_dense: int = 1) -> _lib.FsPlan:
@@ -96,14 +142,11 @@ class RowSplitPlan:
         if st is not None:
             return st
         if self.n_long:
-            counters = torch.zeros(self.n_long * key, dtype=torch.int32, device=self.device)
             nbytes = _lib.partials_bytes(self.value_code, self.accumulate, self.n_chunks, key)
             partials = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=self.device)
         else:
-            counters = torch.zeros(1, dtype=torch.int32, device=self.device)
             partials = torch.empty(8, dtype=torch.uint8, device=self.device)
-            nbytes = 0
-        self._work[key] = (counters, partials)
+        self._work[key] = partials
         sch = self.schedule
         st = _lib.FsPlan(
             accumulate=self.accumulate,
@@ -117,9 +160,11 @@ class RowSplitPlan:
             long_chunk_ptr=_lib.ptr(self.long_chunk_ptr),
             n_chunks=self.n_chunks,
             chunk_long=_lib.ptr(self.chunk_long),
-            counters=counters.data_ptr(),
             partials=partials.data_ptr(),
             partials_bytes=int(partials.numel()),
+            tile_nnz=sch.tile_nnz if key == 1 else 0,
+            n_tiles=self.n_tiles if key == 1 else 0,
+            tile_start=_lib.ptr(self.tile_start) if (key == 1 and self.tile_start is not None) else None,
         )
         self._structs[key] = st
         return st

@@ -1,0 +1,57 @@
+"""How a prepared matrix gets its row-split schedule.
+
+``choose_schedule(op, request, ...)`` resolves a request to a ``Schedule``:
+  * a ``Schedule`` instance is used as given;
+  * "default" is the fixed rule of ``rowsplit``;
+  * "learned" (also what None means) runs the packaged selector for ``op`` on
+    the device -- features from the analysis pass's statistics, MLP forward,
+    argmin -- and falls back to the fixed rule when no model is installed or
+    the values are not fp32;
+  * "exhaustive" is resolved after the prepared object exists
+    (``exhaustive_spmv_schedule`` times every variant).
+Returns (schedule or None, how).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from .rowsplit import Schedule, default_spmm_schedule, default_spmv_schedule
+
+_MODES = ("learned", "default", "exhaustive")
+
+
+def choose_schedule(op, request, data, n_rows, n_cols, stats_dev):
+    nnz = data.numel()
+    if isinstance(request, Schedule):
+        return request, "given"
+    mode = "learned" if request is None else str(request)
+    if mode not in _MODES:
+        raise ValueError(f"schedule must be a Schedule or one of {_MODES}")
+    fallback = (default_spmv_schedule(n_rows, nnz) if op == "spmv"
+                else default_spmm_schedule(n_rows, nnz, 64))
+    if mode == "default" or n_rows == 0:
+        return fallback, "default"
+    if mode == "exhaustive":
+        if op != "spmv":
+            raise ValueError("exhaustive schedule search is offered for SpMV")
+        return fallback, "exhaustive"
+    if data.dtype != torch.float32:
+        return fallback, "default"
+    from .selector import load_default
+    sel = load_default(op, data.device)
+    if sel is None:
+        return fallback, "default"
+    sch, idx = sel.schedule_for(stats_dev, n_rows, n_cols, nnz)
+    return sch, f"learned:{idx}"
+
+
+def exhaustive_spmv_schedule(prepared, reps=5):
+    """Time every SpMV variant on this matrix (random x) and return the fastest."""
+    from .autotune import SPMV_VARIANTS, best_schedule, tune_spmv
+    g = torch.Generator(device=prepared.data.device)
+    g.manual_seed(0)
+    x = (torch.rand(prepared.n_cols, generator=g, device=prepared.data.device) * 2 - 1).to(
+        prepared.data.dtype)
+    times = tune_spmv(prepared, x, reps=reps)
+    return best_schedule(times, SPMV_VARIANTS)[1]

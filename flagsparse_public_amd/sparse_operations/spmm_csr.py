@@ -19,6 +19,7 @@ import torch
 
 from .. import _lib
 from ..rowsplit import RowSplitPlan, Schedule, default_spmm_schedule
+from ..schedule_choice import choose_schedule
 from ._common import (
     SUPPORTED_INDEX_DTYPES,
     SUPPORTED_VALUE_DTYPES,
@@ -103,7 +104,7 @@ def _spmm_validation_metrics(candidate, reference):
 
 # --------------------------------------------------------------- validation
 
-def _prepare_spmm_csr_matrix(data, indices, indptr, shape):
+def _prepare_spmm_csr_matrix(data, indices, indptr, shape, _stats_out=None):
     """Checks and messages of spmm_csr.py:197-255, device checks fused in one
     fs_csr_analyze pass.  Column indices become int32 and indptr int64 as in
     the reference (:247-248)."""
@@ -133,7 +134,10 @@ def _prepare_spmm_csr_matrix(data, indices, indptr, shape):
     data = data.contiguous()
     indices = indices.contiguous()
     indptr = indptr.contiguous()
-    stats = _lib.analyze(_lib.make_csr(data, indices, indptr, n_rows, n_cols), data.device)
+    stats_dev = _lib.analyze_device(_lib.make_csr(data, indices, indptr, n_rows, n_cols), data.device)
+    stats = stats_dev.cpu().tolist()
+    if _stats_out is not None:
+        _stats_out.append(stats_dev)
     if stats[0] != 0:
         raise ValueError("indptr[0] must be 0")
     if stats[1] != nnz:
@@ -417,9 +421,13 @@ def _build_spmm_opt_buckets(row_lengths):
 
 
 def prepare_spmm_csr_opt(data, indices, indptr, shape, schedule=None):
-    """Validate once and cache metadata (spmm_csr.py:480-522)."""
+    """Validate once and cache metadata (spmm_csr.py:480-522).  ``schedule``:
+    a ``Schedule``, "learned" (default when a model is installed) or
+    "default"."""
+    stats_out = []
     (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
-     max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape)
+     max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape, _stats_out=stats_out)
+    schedule, _ = choose_schedule("spmm64", schedule, data, n_rows, n_cols, stats_out[0])
     row_buckets, long_rows = _build_spmm_opt_buckets(row_lengths)
     part_rows, part_starts, part_ends, part_ptr = _build_spmm_opt_split_metadata(
         kernel_indptr, long_rows, 256)

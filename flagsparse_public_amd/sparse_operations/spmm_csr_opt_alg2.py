@@ -16,6 +16,7 @@ import torch
 
 from .. import _lib
 from ..rowsplit import default_spmm_schedule
+from ..schedule_choice import choose_schedule
 from ._common import _INDEX_LIMIT_INT32
 from .spmm_csr import (  # noqa: F401  (re-exported like the reference)
     _launch_spmm,
@@ -206,8 +207,10 @@ def _build_spmm_opt_alg2_buckets(row_lengths, dtype):
 
 def prepare_spmm_csr_opt_alg2(data, indices, indptr, shape, schedule=None):
     """Validate once and cache metadata (spmm_csr_opt_alg2.py:632-653)."""
+    stats_out = []
     (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
-     max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape)
+     max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape, _stats_out=stats_out)
+    schedule, _ = choose_schedule("spmm64", schedule, data, n_rows, n_cols, stats_out[0])
     return PreparedCsrSpmmOptAlg2(
         data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
         n_rows=n_rows, n_cols=n_cols, row_lengths=row_lengths, max_row_nnz=max_row_nnz,

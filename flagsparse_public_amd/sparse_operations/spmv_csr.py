@@ -24,6 +24,7 @@ import torch
 
 from .. import _lib
 from ..rowsplit import RowSplitPlan, Schedule, default_spmv_schedule
+from ..schedule_choice import choose_schedule, exhaustive_spmv_schedule
 from ._common import (
     SUPPORTED_INDEX_DTYPES,
     _INDEX_LIMIT_INT32,
@@ -232,7 +233,8 @@ def _transpose_csr_for_spmv(data, indices, indptr, shape):
             t_indptr.to(out_indptr_dtype).contiguous(), (n_cols, n_rows))
 
 
-def _prepare_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy="auto"):
+def _prepare_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy="auto",
+                             _stats_out=None):
     """Validation with the reference's checks, order and messages
     (spmv_csr.py:497-551); the device-side checks run as one fs_csr_analyze
     pass.  Returns (data, indices, indptr, n_rows, n_cols, row_lengths,
@@ -261,7 +263,10 @@ def _prepare_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy
     indices = indices.contiguous()
     indptr = indptr.contiguous()
     nnz = data.numel()
-    stats = _lib.analyze(_lib.make_csr(data, indices, indptr, n_rows, n_cols), data.device)
+    stats_dev = _lib.analyze_device(_lib.make_csr(data, indices, indptr, n_rows, n_cols), data.device)
+    stats = stats_dev.cpu().tolist()
+    if _stats_out is not None:
+        _stats_out.append(stats_dev)
     if stats[0] != 0:
         raise ValueError("indptr must start at zero")
     if stats[1] != nnz:
@@ -297,8 +302,10 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
     """Validate once and cache metadata (spmv_csr.py:570-649).
 
     ``block_nnz`` / ``max_segments`` are validated and recorded with the
-    reference's auto-doubling rule (:613-626) for API compatibility; the HIP
-    schedule is ``schedule`` (or the default rule).
+    reference's auto-doubling rule (:613-626) for API compatibility.  The HIP
+    schedule is ``schedule``: a ``Schedule``, "learned" (the packaged
+    selector, run on the device; the default when a model is installed),
+    "exhaustive" (time every variant now) or "default" (the fixed rule).
     """
     index_fallback_policy = _normalize_spmv_index_fallback_policy(index_fallback_policy)
     op_code = _normalize_spmv_op(op, transpose=transpose)
@@ -311,9 +318,11 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
         if op_code == SPMV_OP_CONJ_TRANS and _is_complex_dtype(data.dtype):
             data = data.conj().resolve_conj()
         data, indices, indptr, shape = _transpose_csr_for_spmv(data, indices, indptr, shape)
+    stats_out = []
     (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
      max_row_nnz) = _prepare_spmv_csr_matrix(data, indices, indptr, shape,
-                                             index_fallback_policy=index_fallback_policy)
+                                             index_fallback_policy=index_fallback_policy,
+                                             _stats_out=stats_out)
     block_nnz_use = block_nnz
     if max_segments is None:
         max_segments_use = max((max_row_nnz + block_nnz_use - 1) // block_nnz_use, 1)
@@ -322,13 +331,17 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
             max_segments_use = max((max_row_nnz + block_nnz_use - 1) // block_nnz_use, 1)
     else:
         max_segments_use = max_segments
-    sch = schedule or default_spmv_schedule(n_rows, data.numel())
+    sch, how = choose_schedule("spmv", schedule, data, n_rows, n_cols, stats_out[0])
     prepared = PreparedCsrSpmv(
         data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
         n_rows=n_rows, n_cols=n_cols, block_nnz=block_nnz_use, max_segments=max_segments_use,
         max_row_nnz=max_row_nnz, opt_buckets=[], transpose=transpose, op=op_code,
         index_fallback_policy=index_fallback_policy, schedule=sch, row_lengths=row_lengths)
+    if how == "exhaustive":
+        sch = exhaustive_spmv_schedule(prepared)
+        prepared.schedule = sch
     prepared.opt_buckets = _schedule_buckets(sch, n_rows, None)
+    prepared.opt_buckets[0]["chosen_by"] = how
     return prepared
 
 

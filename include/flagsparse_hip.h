@@ -84,13 +84,22 @@ typedef struct fs_csr {
 /*
  * Row-split schedule shared by SpMV and SpMM.  Rows whose length is at most
  * short_max are processed in natural order by groups of group_lanes lanes
- * (SpMV) or by one lane group per row (SpMM).  Longer rows are listed in
- * long_rows and cut into chunks of chunk_nnz entries; one wavefront handles a
- * chunk and the last chunk of a row to finish sums the row's partials in
- * chunk order (deterministic).  counters[n_long] must be zero before the
- * first call; every call leaves them zero again.  partials holds one
- * accumulator per chunk (SpMV) or one accumulator row of n_dense per chunk
- * (SpMM); size it with fs_partials_bytes.
+ * (SpMV) or by one lane group per row and column tile (SpMM).  Longer rows
+ * are listed in long_rows and cut into chunks of chunk_nnz entries; one
+ * wavefront reduces a chunk.  A row that fits one chunk is written by that
+ * wavefront; otherwise every chunk stores its partial (one accumulator for
+ * SpMV, one accumulator row of n_dense for SpMM) and a second small launch
+ * sums each row's partials in chunk order (deterministic; no synchronisation
+ * between workgroups inside a launch).  Size partials with fs_partials_bytes.
+ */
+/*
+ * Tile role (SpMV, tile_nnz > 0): instead of one lane group per short row,
+ * a workgroup takes a tile of consecutive rows [tile_start[t],
+ * tile_start[t+1]), stages their row pointers and the products of all their
+ * entries in LDS (entries loaded fully coalesced, several x gathers in flight
+ * per lane) and sums each row in entry order.  Tiles hold at most 512 rows,
+ * at most 2 * tile_nnz entries and either only rows of length <= short_max
+ * or exactly one longer row (left to the chunk role); short_max <= tile_nnz.
  */
 typedef struct fs_plan {
   int32_t accumulate;      /* FS_ACC_*                                     */
@@ -104,9 +113,11 @@ typedef struct fs_plan {
   const int64_t *long_chunk_ptr; /* [n_long + 1] first chunk of each row   */
   int64_t n_chunks;
   const int32_t *chunk_long;     /* [n_chunks] long-row ordinal of a chunk */
-  int32_t *counters;             /* [n_long] arrival counters              */
   void *partials;                /* chunk partial sums                     */
   int64_t partials_bytes;
+  int64_t tile_nnz;              /* > 0: tile role for short rows (SpMV)   */
+  int64_t n_tiles;
+  const int64_t *tile_start;     /* [n_tiles + 1] first row of each tile   */
 } fs_plan;
 
 /* Bytes of `partials` needed for n_chunks chunks (n_dense = 1 for SpMV). */

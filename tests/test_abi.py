@@ -74,7 +74,7 @@ def _csr(**kw):
 def _plan(**kw):
     base = dict(accumulate=0, group_lanes=8, xcd_remap=0, unroll=4, short_max=64, chunk_nnz=512,
                 n_long=0, long_rows=None, long_chunk_ptr=None, n_chunks=0, chunk_long=None,
-                counters=None, partials=None, partials_bytes=0)
+                partials=None, partials_bytes=0)
     base.update(kw)
     return _lib.FsPlan(**base)
 
@@ -96,9 +96,9 @@ def _spmv(csr, plan):
     (_csr(), _plan(accumulate=5), _lib.FS_ERR_VALUE, "accumulate"),
     (_csr(), _plan(short_max=-1), _lib.FS_ERR_VALUE, "short_max"),
     (_csr(), _plan(n_long=1), _lib.FS_ERR_VALUE, "metadata pointers"),
-    (_csr(), _plan(n_long=1, long_rows=8, long_chunk_ptr=8, chunk_long=8, counters=8,
+    (_csr(), _plan(n_long=1, long_rows=8, long_chunk_ptr=8, chunk_long=8,
                    n_chunks=1, chunk_nnz=100), _lib.FS_ERR_VALUE, "multiple of 64"),
-    (_csr(), _plan(n_long=1, long_rows=8, long_chunk_ptr=8, chunk_long=8, counters=8,
+    (_csr(), _plan(n_long=1, long_rows=8, long_chunk_ptr=8, chunk_long=8,
                    n_chunks=1, chunk_nnz=64, partials=8, partials_bytes=4),
      _lib.FS_ERR_VALUE, "partials buffer too small"),
     (_csr(), _plan(n_chunks=3), _lib.FS_ERR_VALUE, "chunks listed"),

@@ -118,10 +118,16 @@ SCHEDULES = [
         (1, 16, 64, 4, 0), (2, 32, 128, 2, 1), (4, 64, 512, 4, 0), (8, 128, 512, 8, 1),
         (16, 256, 1024, 4, 0), (32, 512, 2048, 4, 1), (64, 4096, 4096, 8, 0), (8, 0, 64, 4, 0),
     ]
+] + [
+    Schedule(tile_nnz=t, short_max=sm, chunk_nnz=ch, unroll=u, xcd_remap=xr)
+    for t, sm, ch, u, xr in [
+        (256, 64, 512, 4, 0), (1024, 1024, 2048, 8, 1), (2048, 2048, 4096, 2, 0),
+        (2048, 256, 8192, 8, 1), (512, 0, 64, 4, 0),
+    ]
 ]
 
 
-@pytest.mark.parametrize("sch", SCHEDULES, ids=lambda s: f"S{s.group_lanes}_m{s.short_max}_c{s.chunk_nnz}_u{s.unroll}_x{s.xcd_remap}")
+@pytest.mark.parametrize("sch", SCHEDULES, ids=lambda s: f"S{s.group_lanes}_t{s.tile_nnz}_m{s.short_max}_c{s.chunk_nnz}_u{s.unroll}_x{s.xcd_remap}")
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
 @pytest.mark.parametrize("ptr_dtype", [torch.int32, torch.int64], ids=["p32", "p64"])
 def test_spmv_schedules_powerlaw(orc, sch, dtype, ptr_dtype):
@@ -159,6 +165,37 @@ def test_spmv_long_rows_compensated_fp32(orc):
     cond = orc.spmv_abs(ip, cols, vals, x)
     y = flagsparse_spmv_csr(to_dev(vals), to_dev(cols), to_dev(ip), to_dev(x), (n, m), use_opt=True)
     assert row_rel_err(host64(y), y64, cond).max() <= 1e-6
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.complex64], ids=["f32", "f64", "c64"])
+def test_spmv_tiles_many_empty_rows(orc, dtype):
+    """Tiles are capped at 512 rows: a matrix of mostly empty rows with a few
+    dense and long ones."""
+    rng = np.random.default_rng(17)
+    n, m = 70000, 5000
+    lens = np.zeros(n, np.int64)
+    hot = rng.choice(n, 300, replace=False)
+    lens[hot] = rng.integers(1, 40, 300)
+    lens[hot[:5]] = [3000, 4500, 2049, 2048, 9000]
+    ip = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    cols = np.concatenate([np.sort(rng.choice(m, l, replace=False)) for l in lens if l]).astype(np.int32)
+    npd = {torch.float32: np.float32, torch.float64: np.float64, torch.complex64: np.complex64}[dtype]
+    vals = rng.uniform(-1, 1, ip[-1]).astype(npd)
+    x = rng.uniform(-1, 1, m).astype(npd)
+    if dtype.is_complex:
+        vals = (vals + 1j * rng.uniform(-1, 1, ip[-1])).astype(npd)
+        x = (x + 1j * rng.uniform(-1, 1, m)).astype(npd)
+    y64 = orc.spmv(ip, cols, vals, x)
+    cond = orc.spmv_abs(ip, cols, vals, x)
+    for sch in (Schedule(tile_nnz=2048, short_max=2048, chunk_nnz=4096, unroll=8),
+                Schedule(tile_nnz=256, short_max=256, chunk_nnz=256, unroll=4)):
+        p = prepare_spmv_csr(to_dev(vals), to_dev(cols), to_dev(ip), (n, m), schedule=sch)
+        for use_opt in (False, True):
+            y = flagsparse_spmv_csr(x=to_dev(x), prepared=p, use_opt=use_opt)
+            got = y.cpu().to(torch.complex128).numpy() if dtype.is_complex else host64(y)
+            tol = 1e-12 if dtype == torch.float64 else 1e-6
+            assert row_rel_err(got, y64, cond).max() <= tol
 
 
 def test_spmv_alpha_beta_and_out():
@@ -279,8 +316,11 @@ def test_spmv_validation_errors():
         flagsparse_spmv_csr(d, i, torch.tensor([1, 2, 3], device=DEV), x, (2, 2))
     with pytest.raises(ValueError, match=r"indptr\[-1\] must equal nnz"):
         flagsparse_spmv_csr(d, i, torch.tensor([0, 2, 2], device=DEV), x, (2, 2))
-    with pytest.raises(ValueError, match="non-decreasing"):
+    # the reference checks indptr[-1] == nnz before monotonicity
+    with pytest.raises(ValueError, match=r"indptr\[-1\] must equal nnz"):
         flagsparse_spmv_csr(d, i, torch.tensor([0, 3, 2], device=DEV), x, (2, 2))
+    with pytest.raises(ValueError, match="non-decreasing"):
+        flagsparse_spmv_csr(d, i, torch.tensor([0, 3, 1, 3], device=DEV), x, (3, 2))
     with pytest.raises(IndexError):
         flagsparse_spmv_csr(d, torch.tensor([0, 3, 1], dtype=torch.int32, device=DEV), p, x, (2, 2))
     with pytest.raises(TypeError, match="x dtype"):

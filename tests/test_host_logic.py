@@ -214,10 +214,31 @@ def test_error_metrics():
 
 
 def test_default_schedules():
-    s = rowsplit.default_spmv_schedule(4_000_000, 48_000_000)
+    s = rowsplit.rowgroup_spmv_schedule(4_000_000, 48_000_000)
     assert s.group_lanes == 8 and s.short_max == 128 and s.chunk_nnz % 64 == 0
-    assert rowsplit.default_spmv_schedule(10, 10).group_lanes == 1
-    assert rowsplit.default_spmv_schedule(10, 100000).group_lanes == 64
+    assert rowsplit.rowgroup_spmv_schedule(10, 10).group_lanes == 1
+    assert rowsplit.rowgroup_spmv_schedule(10, 100000).group_lanes == 64
+    d = rowsplit.default_spmv_schedule(4_000_000, 48_000_000)
+    assert d.tile_nnz > 0 and d.short_max <= d.tile_nnz and d.chunk_nnz % 64 == 0
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tile_starts_invariants(seed):
+    rng = np.random.default_rng(seed)
+    lens = rng.poisson(3, 5000)
+    lens[rng.random(5000) < 0.01] = rng.integers(100, 20000, int((rng.random(5000) < 0.01).sum()) or 1)[0]
+    lens[rng.random(5000) < 0.2] = 0
+    t = torch.from_numpy(lens)
+    for short_max, tile in ((64, 256), (1024, 1024), (2048, 2048)):
+        ts = rowsplit.tile_starts(t, short_max, tile).tolist()
+        assert ts[0] == 0 and ts[-1] == lens.size and ts == sorted(set(ts))
+        for a, b in zip(ts[:-1], ts[1:]):
+            seg = lens[a:b]
+            assert b - a <= rowsplit.TILE_ROWS
+            if (seg > short_max).any():
+                assert b - a == 1  # a long row is a tile of its own
+            else:
+                assert seg.sum() < tile + short_max
 
 
 def test_synthetic_generators_deterministic_and_sorted():
@@ -291,3 +312,10 @@ def test_batch_pack_validates_on_host():
     ents[1].group_lanes = 4
     ents[1].A.value_type = _lib.FS_F64
     assert lib.fs_batch_pack(ents, 2, ctypes.addressof(buf)) == _lib.FS_ERR_TYPE
+
+
+def test_batched_cuts_small_matrices():
+    from flagsparse_public_amd.batched import _cuts
+    ip = torch.tensor([0, 2, 3, 3])
+    assert _cuts(ip, 12.0, 96 * 1024, 2048, 4096).tolist() == [0, 3]
+    assert _cuts(torch.tensor([0]), 12.0, 100, 4).tolist() == [0]
