@@ -76,6 +76,22 @@ def cpu_baseline(ip, cols, vals, x, nbytes, budget_s=10.0):
                        f"fp32 algorithmic bytes"}, y64)
 
 
+def calibrate_copy(mib, reps):
+    """Plain device-to-device copies of a known size: the reference point that
+    turns rocprofv3 FETCH_SIZE / WRITE_SIZE readings into bytes."""
+    import torch
+    dev = torch.device("cuda", 0)
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}),
+          flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,7 +100,11 @@ def main(argv=None):
     ap.add_argument("--path", choices=["opt", "default"], default="opt")
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="only run device copies of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
+    if args.calibrate_copy_mb:
+        return calibrate_copy(args.calibrate_copy_mb, max(args.steps, 1))
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(the SpMV row-block sharding is described, not run)", file=sys.stderr)
@@ -155,9 +175,10 @@ def main(argv=None):
             "workload": "spmv_powerlaw_4m",
             "n_rows": n, "nnz": nnz, "max_row_nnz": int(np.diff(ip).max()),
             "index_dtype": "int32", "path": args.path,
-            "schedule": {"group_lanes": sch.group_lanes, "short_max": sch.short_max,
-                         "chunk_nnz": sch.chunk_nnz, "unroll": sch.unroll,
-                         "xcd_remap": sch.xcd_remap},
+            "schedule": {"group_lanes": sch.group_lanes, "tile_nnz": sch.tile_nnz,
+                         "short_max": sch.short_max, "chunk_nnz": sch.chunk_nnz,
+                         "unroll": sch.unroll, "xcd_remap": sch.xcd_remap,
+                         "chosen_by": prepared.opt_buckets[0].get("chosen_by")},
             "gflops": round(2.0 * nnz / (per_launch_ms * 1e-3) / 1e9, 2),
             "compulsory_bytes": (n + 1) * 4 + nnz * 8 + n * 4 + n * 4,
         },
@@ -169,6 +190,7 @@ def main(argv=None):
             "frac": round(gbs / HBM_PEAK_GBS, 4),
             "traffic": _traffic_from_profile(),
             "algorithmic_bytes_per_launch": nbytes,
+            # one step = the SpMV kernel plus the small long-row fix-up launch
             "avg_launch_us": round(per_launch_ms * 1e3, 3),
         },
     }

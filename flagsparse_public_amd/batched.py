@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .rowsplit import default_spmv_schedule
+from .rowsplit import rowgroup_spmv_schedule
 from .sparse_operations.spmv_csr import _prepare_spmv_csr_matrix
 
 
@@ -104,7 +104,7 @@ class BatchedCsr:
                 n_dense = int(B.shape[1])
                 C = torch.zeros((n_rows, n_dense), dtype=torch.float32, device=dev)
             nnz = data.numel()
-            s = m.get("group_lanes") or default_spmv_schedule(n_rows, nnz).group_lanes
+            s = m.get("group_lanes") or rowgroup_spmv_schedule(n_rows, nnz).group_lanes
             e = self.entries[i]
             e.A = _lib.make_csr(data, indices, indptr, n_rows, n_cols)
             e.x, e.y = _lib.ptr(x), _lib.ptr(y)

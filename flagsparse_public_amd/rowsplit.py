@@ -31,6 +31,8 @@ class Schedule:
     xcd_remap: int = 0        # 1: contiguous row ranges per XCD
     col_tile: int = 0         # SpMM dense columns per lane group (0: all)
     tile_nnz: int = 0         # SpMV: > 0 selects the tile role for short rows
+    x_slice_cols: int = 0     # SpMV native path: > 0 cuts the columns into slices of
+                              # at most this many (one launch per slice, x slice in L2)
 
     def key(self) -> tuple:
         return dataclasses.astuple(self)
@@ -46,10 +48,12 @@ def _pow2_ceil(v: float) -> int:
 TILE_ROWS = 512  # rows per tile at most (the kernel's LDS row-pointer slots)
 
 
-def default_spmv_schedule(n_rows: int, nnz: int) -> Schedule:
+def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None) -> Schedule:
     """Fixed rule used when no selector is supplied: tiles of ~2048 entries for
-    rows up to 2048 entries, longer rows in 4096-entry chunks."""
-    return Schedule(group_lanes=8, short_max=2048, chunk_nnz=4096, unroll=8, tile_nnz=2048)
+    rows up to 2048 entries (1024 for complex values, whose fp64 row sums take
+    16 bytes of LDS each), longer rows in 8192-entry chunks."""
+    t = 1024 if (value_dtype is not None and value_dtype.is_complex) else 2048
+    return Schedule(group_lanes=8, short_max=t, chunk_nnz=8192, unroll=4, tile_nnz=t)
 
 
 def rowgroup_spmv_schedule(n_rows: int, nnz: int) -> Schedule:

@@ -25,6 +25,7 @@ import torch
 from .. import _lib
 from ..rowsplit import RowSplitPlan, Schedule, default_spmv_schedule
 from ..schedule_choice import choose_schedule, exhaustive_spmv_schedule
+from ..slicing import SlicedCsr
 from ._common import (
     SUPPORTED_INDEX_DTYPES,
     _INDEX_LIMIT_INT32,
@@ -131,7 +132,8 @@ class PreparedCsrSpmv:
         return self._csr
 
     def plan(self, accumulate, schedule=None):
-        sch = schedule or self.schedule or default_spmv_schedule(self.n_rows, self.data.numel())
+        sch = schedule or self.schedule or default_spmv_schedule(self.n_rows, self.data.numel(),
+                                                                 self.data.dtype)
         key = (int(accumulate), sch.key())
         p = self._plans.get(key)
         if p is None:
@@ -355,6 +357,18 @@ def _launch_spmv(prepared, x, accumulate, out=None, alpha=1.0, beta=0.0, schedul
     y = out if out is not None else torch.empty(prepared.n_rows, dtype=prepared.data.dtype,
                                                  device=prepared.data.device)
     if prepared.n_rows == 0:
+        return y
+    sch = schedule or prepared.schedule
+    if (sch is not None and sch.x_slice_cols > 0 and prepared.n_cols > sch.x_slice_cols
+            and accumulate == _lib.FS_ACC_NATIVE):
+        key = ("sliced", sch.key())
+        sliced = prepared._plans.get(key)
+        if sliced is None:
+            sliced = SlicedCsr(prepared.data, prepared.kernel_indices, prepared.kernel_indptr,
+                               prepared.n_rows, prepared.n_cols, sch.x_slice_cols, sch,
+                               accumulate)
+            prepared._plans[key] = sliced
+        sliced.run(x, y, alpha, beta, prepared.data.device)
         return y
     plan = prepared.plan(accumulate, schedule)
     _lib.spmv(prepared.csr_struct(), x, y, alpha, beta, plan.struct(1), prepared.data.device)

@@ -124,10 +124,14 @@ SCHEDULES = [
         (256, 64, 512, 4, 0), (1024, 1024, 2048, 8, 1), (2048, 2048, 4096, 2, 0),
         (2048, 256, 8192, 8, 1), (512, 0, 64, 4, 0),
     ]
+] + [
+    # column slices (native path): 7 and 3 slices of the 20000 columns
+    Schedule(tile_nnz=1024, short_max=1024, chunk_nnz=2048, unroll=4, x_slice_cols=3000),
+    Schedule(group_lanes=8, short_max=128, chunk_nnz=512, unroll=4, x_slice_cols=7000),
 ]
 
 
-@pytest.mark.parametrize("sch", SCHEDULES, ids=lambda s: f"S{s.group_lanes}_t{s.tile_nnz}_m{s.short_max}_c{s.chunk_nnz}_u{s.unroll}_x{s.xcd_remap}")
+@pytest.mark.parametrize("sch", SCHEDULES, ids=lambda s: f"S{s.group_lanes}_t{s.tile_nnz}_m{s.short_max}_c{s.chunk_nnz}_u{s.unroll}_x{s.xcd_remap}_sl{s.x_slice_cols}")
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
 @pytest.mark.parametrize("ptr_dtype", [torch.int32, torch.int64], ids=["p32", "p64"])
 def test_spmv_schedules_powerlaw(orc, sch, dtype, ptr_dtype):
