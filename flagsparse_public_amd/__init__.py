@@ -13,7 +13,11 @@ _OPS_EXPORTS = {
     "PreparedCsrSpmmOptAlg2",
     "SUPPORTED_INDEX_DTYPES",
     "SUPPORTED_VALUE_DTYPES",
+    "benchmark_spmm_case",
+    "benchmark_spmm_opt_alg2_case",
     "benchmark_spmm_opt_case",
+    "benchmark_spmv_case",
+    "comprehensive_spmm_test",
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
     "flagsparse_spmm_csr_opt_alg2",

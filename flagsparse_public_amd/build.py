@@ -6,6 +6,10 @@ Each ``csrc/*.hip`` file is compiled to an object in ``csrc/_build/`` (in
 parallel), then linked into ``flagsparse_public_amd/libflagsparse_hip.so``.
 Packed-fp32 instructions are disabled for all device code
 (``-target-feature -packed-fp32-ops``).
+
+``csrc/vendor_rocsparse.cpp`` (host code only) is linked on its own into
+``libfs_vendor.so`` against rocSPARSE: the vendor baseline the benchmark
+harness times beside the library; the product library does not link it.
 """
 
 from __future__ import annotations
@@ -24,6 +28,7 @@ BUILD_DIR = os.path.join(CSRC, "_build")
 LIB_NAME = "libflagsparse_hip.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = "gfx950"
+VENDOR_LIB_PATH = os.path.join(PKG_DIR, "libfs_vendor.so")
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
@@ -90,7 +95,23 @@ def build(jobs: int | None = None, force: bool = True, verbose: bool = False) ->
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stdout}{proc.stderr}")
     os.replace(tmp, LIB_PATH)
+    build_vendor()
     return LIB_PATH
+
+
+def build_vendor() -> str:
+    """Host-only C++ wrapper over rocSPARSE (benchmark baseline)."""
+    src = os.path.join(CSRC, "vendor_rocsparse.cpp")
+    tmp = VENDOR_LIB_PATH + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-D__HIP_PLATFORM_AMD__",
+           f"-I{os.path.join(ROCM, 'include')}", f"-I{os.path.join(PKG_DIR, '..', 'include')}",
+           src, "-o", tmp, f"-L{os.path.join(ROCM, 'lib')}", "-lrocsparse", "-lamdhip64",
+           f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"vendor baseline build failed:\n{proc.stdout}{proc.stderr}")
+    os.replace(tmp, VENDOR_LIB_PATH)
+    return VENDOR_LIB_PATH
 
 
 def main(argv=None) -> int:

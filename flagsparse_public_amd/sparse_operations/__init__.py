@@ -17,9 +17,11 @@ from .spmm_csr import (
 )
 from .spmm_csr_opt_alg2 import (
     PreparedCsrSpmmOptAlg2,
+    benchmark_spmm_opt_alg2_case,
     flagsparse_spmm_csr_opt_alg2,
     prepare_spmm_csr_opt_alg2,
 )
+from .benchmarks import benchmark_spmm_case, benchmark_spmv_case, comprehensive_spmm_test
 
 __all__ = [
     "PreparedCsrSpmv",
@@ -27,7 +29,11 @@ __all__ = [
     "PreparedCsrSpmmOptAlg2",
     "SUPPORTED_INDEX_DTYPES",
     "SUPPORTED_VALUE_DTYPES",
+    "benchmark_spmm_case",
+    "benchmark_spmm_opt_alg2_case",
     "benchmark_spmm_opt_case",
+    "benchmark_spmv_case",
+    "comprehensive_spmm_test",
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
     "flagsparse_spmm_csr_opt_alg2",

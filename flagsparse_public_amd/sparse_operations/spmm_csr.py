@@ -540,7 +540,7 @@ def benchmark_spmm_opt_case(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32
                             value_dtype=torch.float32, index_dtype=torch.int32, warmup=20,
                             iters=200, run_cusparse=True):
     """Base vs opt timing and error against the promoted torch reference
-    (spmm_csr.py:2226-2348).  ``cu_ms`` is None: there is no cuSPARSE."""
+    (spmm_csr.py:2226-2348).  ``cu_ms`` times the vendor library, rocSPARSE."""
     if value_dtype not in (torch.float32, torch.float64):
         raise TypeError("benchmark_spmm_opt_case only supports float32 and float64")
     device = torch.device("cuda")
@@ -572,15 +572,21 @@ def benchmark_spmm_opt_case(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32
         pt_ms = None
     err_base = _spmm_opt_reference_error(base, ref, value_dtype)
     err_opt = _spmm_opt_reference_error(opt, ref, value_dtype)
+    cu_ms = None
+    if run_cusparse:
+        from .. import vendor
+        _, run, _reason = vendor.spmm_baseline(data, indices, indptr, shape, B)
+        if run is not None:
+            _, cu_ms = _benchmark_cuda_op(run, warmup, iters)
     return {
         "parameters": {"n_rows": n_rows, "n_cols": n_cols, "nnz": nnz,
                        "n_dense_cols": n_dense_cols, "value_dtype": str(value_dtype),
                        "index_dtype": str(index_dtype)},
         "performance": {"base_ms": base_ms, "base_first_call_ms": base_first, "opt_ms": opt_ms,
-                        "opt_first_call_ms": opt_first, "pt_ms": pt_ms, "cu_ms": None,
+                        "opt_first_call_ms": opt_first, "pt_ms": pt_ms, "cu_ms": cu_ms,
                         "opt_vs_base": base_ms / opt_ms if opt_ms > 0 else None,
                         "opt_vs_pt": pt_ms / opt_ms if pt_ms is not None and opt_ms > 0 else None,
-                        "opt_vs_cu": None},
+                        "opt_vs_cu": cu_ms / opt_ms if cu_ms is not None and opt_ms > 0 else None},
         "verification": {"err_base": err_base, "err_opt": err_opt, "base_ok": err_base <= 1.0,
                          "opt_ok": err_opt <= 1.0, "status": "PASS" if err_opt <= 1.0 else "FAIL"},
         "backend_status": {"long_row_fallback_used": bool(prepared.long_row_fallback_only),
@@ -595,6 +601,144 @@ __all__ = [
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
     "prepare_spmm_csr_opt",
+    "benchmark_spmm_case",
     "benchmark_spmm_opt_case",
+    "comprehensive_spmm_test",
     "Schedule",
 ]
+
+
+def benchmark_spmm_case(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32,
+                        value_dtype=torch.float32, index_dtype=torch.int32, warmup=20, iters=200,
+                        block_n=None, block_nnz=None, max_segments=None, run_cusparse=True):
+    """ALG1 SpMM vs torch.sparse.mm and the vendor library on one random matrix;
+    parameters and result layout of spmm_csr.py:2351-2564.  The vendor
+    library is rocSPARSE (its numbers fill the ``cusparse_*`` fields)."""
+    from .. import vendor
+    device = torch.device("cuda")
+    data, indices, indptr = _build_random_csr(n_rows, n_cols, nnz, value_dtype, index_dtype, device)
+    B = _build_random_dense((n_cols, n_dense_cols), value_dtype, device)
+    shape = (n_rows, n_cols)
+    max_row_nnz = int(torch.max(indptr[1:] - indptr[:-1]).item()) if n_rows > 0 else 0
+    launch = _resolve_spmm_alg1_launch_config(n_dense_cols, max_row_nnz, block_n=block_n,
+                                              block_nnz=block_nnz, max_segments=max_segments)
+    kwargs = dict(data=data, indices=indices, indptr=indptr, B=B, shape=shape,
+                  block_n=launch["block_n"], block_nnz=launch["block_nnz"],
+                  max_segments=launch["max_segments"], return_time=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    flagsparse_spmm_csr(**kwargs)
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t0) * 1000.0
+    hip_C, hip_ms = _benchmark_cuda_op(lambda: flagsparse_spmm_csr(**kwargs), warmup, iters)
+
+    indptr64, indices64 = indptr.to(torch.int64), indices.to(torch.int64)
+    pytorch_reason, pytorch_ms, pytorch_format = None, None, "CSR"
+    promote = {torch.float16: torch.float32, torch.bfloat16: torch.float32,
+               torch.float32: torch.float64, torch.complex64: torch.complex128}
+    ref_dtype = promote.get(value_dtype, value_dtype)
+    try:
+        csr_pt = torch.sparse_csr_tensor(indptr64, indices64, data, size=shape, device=device)
+        pytorch_op = lambda: torch.sparse.mm(csr_pt, B)  # noqa: E731
+        csr_ref = torch.sparse_csr_tensor(indptr64, indices64, data.to(ref_dtype), size=shape,
+                                          device=device)
+        expected = torch.sparse.mm(csr_ref, B.to(ref_dtype)).to(value_dtype)
+    except Exception as exc:
+        pytorch_format = "COO"
+        pytorch_reason = f"CSR fallback: {exc}"
+        rows = torch.repeat_interleave(torch.arange(n_rows, device=device, dtype=torch.int64),
+                                       indptr64[1:] - indptr64[:-1])
+        coo = torch.sparse_coo_tensor(torch.stack([rows, indices64]), data, shape,
+                                      device=device).coalesce()
+        pytorch_op = lambda: torch.sparse.mm(coo, B)  # noqa: E731
+        expected = torch.sparse.mm(coo.to(ref_dtype), B.to(ref_dtype)).to(value_dtype)
+    pytorch_values = expected
+    try:
+        pytorch_values, pytorch_ms = _benchmark_cuda_op(pytorch_op, warmup, iters)
+    except Exception as exc:
+        pytorch_reason = str(exc) if pytorch_reason is None else f"{pytorch_reason}; timing: {exc}"
+    hip_metrics = _spmm_validation_metrics(hip_C, expected)
+
+    vendor_ms = vendor_values = vendor_metrics = vendor_match = None
+    vendor_reason = None
+    if run_cusparse:
+        _, run, vendor_reason = vendor.spmm_baseline(data, indices, indptr, shape, B)
+        if run is not None:
+            try:
+                vendor_values, vendor_ms = _benchmark_cuda_op(run, warmup, iters)
+                vendor_values = vendor_values.clone()
+                vendor_metrics = _spmm_validation_metrics(vendor_values, expected)
+                vendor_match = vendor_metrics["strict_allclose_match"]
+            except Exception as exc:
+                vendor_reason = str(exc)
+    threshold = _spmm_relative_threshold(value_dtype)
+
+    def vm(key):
+        return vendor_metrics[key] if vendor_metrics is not None else None
+
+    return {
+        "parameters": {
+            "format": "csr", "n_rows": n_rows, "n_cols": n_cols, "nnz": nnz,
+            "n_dense_cols": n_dense_cols, "value_dtype": str(value_dtype),
+            "index_dtype": str(index_dtype), "warmup": warmup, "iters": iters,
+            "block_n": launch["block_n"], "block_nnz": launch["block_nnz"],
+            "max_segments": launch["max_segments"],
+            "required_segments": launch["required_segments"],
+            "alg1_warp_size": launch["warp_size"], "alg1_factor": launch["factor"],
+            "auto_max_segments": launch["auto_max_segments"], "run_cusparse": run_cusparse,
+        },
+        "performance": {
+            "pytorch_ms": pytorch_ms,
+            "triton_ms": hip_ms,
+            "triton_first_call_ms": first_ms,
+            "cusparse_ms": vendor_ms,
+            "triton_speedup_vs_pytorch": (pytorch_ms / hip_ms
+                                          if pytorch_ms is not None and hip_ms > 0 else None),
+            "triton_speedup_vs_cusparse": (vendor_ms / hip_ms
+                                           if vendor_ms is not None and hip_ms > 0 else None),
+        },
+        "verification": {
+            "triton_match_reference": hip_metrics["strict_allclose_match"],
+            "triton_match_pytorch": hip_metrics["strict_allclose_match"],
+            "triton_max_error": hip_metrics["max_abs_error"],
+            "triton_max_abs_error": hip_metrics["max_abs_error"],
+            "triton_max_relative_error": hip_metrics["max_relative_error"],
+            "triton_sum_relative_error": hip_metrics["sum_relative_error"],
+            "triton_relative_threshold": hip_metrics["relative_threshold"],
+            "triton_strict_allclose_match": hip_metrics["strict_allclose_match"],
+            "pytorch_match_reference": True,
+            "pytorch_max_error": 0.0,
+            "pytorch_max_abs_error": 0.0,
+            "pytorch_max_relative_error": 0.0,
+            "pytorch_sum_relative_error": 0.0,
+            "pytorch_relative_threshold": threshold,
+            "cusparse_match_reference": vendor_match,
+            "cusparse_match_pytorch": vendor_match,
+            "cusparse_max_error": vm("max_abs_error"),
+            "cusparse_max_abs_error": vm("max_abs_error"),
+            "cusparse_max_relative_error": vm("max_relative_error"),
+            "cusparse_sum_relative_error": vm("sum_relative_error"),
+            "cusparse_relative_threshold": (vendor_metrics["relative_threshold"]
+                                            if vendor_metrics is not None else threshold),
+            "cusparse_strict_allclose_match": vm("strict_allclose_match"),
+        },
+        "backend_status": {
+            "pytorch_unavailable_reason": pytorch_reason,
+            "pytorch_sparse_format": pytorch_format,
+            "cusparse_unavailable_reason": vendor_reason,
+            "vendor_library": "rocSPARSE",
+            "flagsparse_internal_route": "csr-alg1",
+        },
+        "samples": {"pytorch": pytorch_values, "triton": hip_C, "reference": expected,
+                    "cusparse": vendor_values},
+    }
+
+
+def comprehensive_spmm_test(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32,
+                            dtype=torch.float32, index_dtype=torch.int32, warmup=20, iters=200,
+                            block_n=None, block_nnz=None, max_segments=None, run_cusparse=True):
+    """One-configuration SpMM benchmark entry (spmm_csr.py:2565-2593)."""
+    return benchmark_spmm_case(n_rows=n_rows, n_cols=n_cols, nnz=nnz, n_dense_cols=n_dense_cols,
+                               value_dtype=dtype, index_dtype=index_dtype, warmup=warmup,
+                               iters=iters, block_n=block_n, block_nnz=block_nnz,
+                               max_segments=max_segments, run_cusparse=run_cusparse)

@@ -328,8 +328,76 @@ def _spmm_opt_alg2_reference_error(candidate, reference, value_dtype):
                            (atol + rtol * torch.abs(reference))).item())
 
 
+def benchmark_spmm_opt_alg2_case(n_rows=4096, n_cols=4096, nnz=65536, n_dense_cols=32,
+                                 value_dtype=torch.float32, index_dtype=torch.int32, warmup=20,
+                                 iters=200, run_cusparse=True):
+    """Base, opt and alg2 SpMM against torch sparse and the vendor library
+    (rocSPARSE in the ``cusparse_*`` fields); spmm_csr_opt_alg2.py:862-987."""
+    from .. import vendor
+    from ._common import _benchmark_cuda_op, _build_random_csr, _build_random_dense
+    from .spmm_csr import flagsparse_spmm_csr, flagsparse_spmm_csr_opt, prepare_spmm_csr_opt
+    if value_dtype not in SUPPORTED_SPMM_OPT_ALG2_DTYPES:
+        raise TypeError("benchmark_spmm_opt_alg2_case only supports float32 and float64")
+    device = torch.device("cuda")
+    data, indices, indptr = _build_random_csr(n_rows, n_cols, nnz, value_dtype, index_dtype, device)
+    B = _build_random_dense((n_cols, n_dense_cols), value_dtype, device)
+    shape = (n_rows, n_cols)
+    prepared_opt = prepare_spmm_csr_opt(data, indices, indptr, shape)
+    prepared_alg2 = prepare_spmm_csr_opt_alg2(data, indices, indptr, shape)
+    base_values, base_ms = _benchmark_cuda_op(
+        lambda: flagsparse_spmm_csr(data, indices, indptr, B, shape), warmup, iters)
+    opt_values, opt_ms = _benchmark_cuda_op(
+        lambda: flagsparse_spmm_csr_opt(B=B, prepared=prepared_opt), warmup, iters)
+    alg2_values, alg2_ms = _benchmark_cuda_op(
+        lambda: flagsparse_spmm_csr_opt_alg2(B=B, prepared=prepared_alg2), warmup, iters)
+    ip64, ix64 = indptr.to(torch.int64), indices.to(torch.int64)
+    ref_dtype = torch.float64 if value_dtype == torch.float32 else value_dtype
+    csr_ref = torch.sparse_csr_tensor(ip64, ix64, data.to(ref_dtype), size=shape, device=device)
+    torch_ref = torch.sparse.mm(csr_ref, B.to(ref_dtype)).to(value_dtype)
+    torch_ms = None
+    try:
+        pt = torch.sparse_csr_tensor(ip64, ix64, data, size=shape, device=device)
+        _, torch_ms = _benchmark_cuda_op(lambda: torch.sparse.mm(pt, B), warmup, iters)
+    except Exception:
+        torch_ms = None
+    vendor_values = vendor_ms = vendor_reason = None
+    if run_cusparse:
+        _, run, vendor_reason = vendor.spmm_baseline(data, indices, indptr, shape, B)
+        if run is not None:
+            try:
+                vendor_values, vendor_ms = _benchmark_cuda_op(run, warmup, iters)
+                vendor_values = vendor_values.clone()
+            except Exception as exc:
+                vendor_reason = str(exc)
+
+    def err(a, b):
+        return _spmm_opt_alg2_reference_error(a, b, value_dtype) if b is not None else None
+
+    return {
+        "parameters": {"n_rows": n_rows, "n_cols": n_cols, "nnz": nnz,
+                       "n_dense_cols": n_dense_cols, "value_dtype": str(value_dtype),
+                       "index_dtype": str(index_dtype)},
+        "performance": {"base_ms": base_ms, "opt_ms": opt_ms, "opt_alg2_ms": alg2_ms,
+                        "torch_ms": torch_ms, "cusparse_ms": vendor_ms},
+        "verification": {
+            "base_vs_torch_err": err(base_values, torch_ref),
+            "opt_vs_torch_err": err(opt_values, torch_ref),
+            "opt_alg2_vs_torch_err": err(alg2_values, torch_ref),
+            "base_vs_cusparse_err": err(base_values, vendor_values),
+            "opt_vs_cusparse_err": err(opt_values, vendor_values),
+            "opt_alg2_vs_cusparse_err": err(alg2_values, vendor_values),
+        },
+        "backend_status": {"cusparse_unavailable_reason": vendor_reason,
+                           "vendor_library": "rocSPARSE"},
+        "samples": {"base_triton": base_values, "opt_triton": opt_values,
+                    "opt_alg2_triton": alg2_values, "torch_ref": torch_ref,
+                    "cusparse_ref": vendor_values},
+    }
+
+
 __all__ = [
     "SUPPORTED_SPMM_OPT_ALG2_DTYPES",
+    "benchmark_spmm_opt_alg2_case",
     "PreparedCsrSpmmOptAlg2",
     "prepare_spmm_csr_opt_alg2",
     "flagsparse_spmm_csr_opt_alg2",

@@ -176,7 +176,7 @@ def test_spmv_tiles_many_empty_rows(orc, dtype):
     """Tiles are capped at 512 rows: a matrix of mostly empty rows with a few
     dense and long ones."""
     rng = np.random.default_rng(17)
-    n, m = 70000, 5000
+    n, m = 70000, 12000
     lens = np.zeros(n, np.int64)
     hot = rng.choice(n, 300, replace=False)
     lens[hot] = rng.integers(1, 40, 300)
