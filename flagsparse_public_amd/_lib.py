@@ -27,6 +27,7 @@ FS_I32, FS_I64 = 0, 1
 FS_ACC_NATIVE, FS_ACC_WIDE = 0, 1
 FS_STORE_BF16X2, FS_STORE_BF16, FS_STORE_F16_ROWSCALE = 1, 2, 3
 FS_STATS_LEN = 16
+FS_TILE_ROWS, FS_TILE_ENTRIES = 0, 1
 
 _COMPLEX32 = getattr(torch, "complex32", None) or getattr(torch, "chalf", None)
 
@@ -76,6 +77,10 @@ class FsPlan(ctypes.Structure):
         ("tile_nnz", ctypes.c_int64),
         ("n_tiles", ctypes.c_int64),
         ("tile_start", ctypes.c_void_p),
+        ("tile_mode", ctypes.c_int32),
+        ("n_groups", ctypes.c_int32),
+        ("group_blocks", ctypes.c_int64),
+        ("group_tiles", ctypes.c_void_p),
     ]
 
 
@@ -91,6 +96,19 @@ class FsBatchEntry(ctypes.Structure):
         ("ldc", ctypes.c_int64),
         ("group_lanes", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+    ]
+
+
+class FsSlices(ctypes.Structure):
+    _fields_ = [
+        ("value_type", ctypes.c_int32),
+        ("n_slices", ctypes.c_int32),
+        ("n_rows", ctypes.c_int64),
+        ("n_compact", ctypes.c_int64),
+        ("compact_row", ctypes.c_void_p),
+        ("bounds", ctypes.c_void_p),
+        ("rows_per_block", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 
@@ -110,6 +128,9 @@ _lib_lock = threading.Lock()
 # Every symbol the header declares, with its ctypes signature.
 _SIGNATURES = {
     "fs_partials_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64]),
+    "fs_tile_carry_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int64]),
+    "fs_sliced_combine": (ctypes.c_int, [ctypes.POINTER(FsSlices), ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_void_p]),
     "fs_spmv_csr": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_double, ctypes.c_double, ctypes.POINTER(FsPlan),
                                    ctypes.c_void_p]),
@@ -329,6 +350,15 @@ def spmv_split(csr: FsCsr, fmt: int, hi, lo, row_scale, x, y, alpha: float, beta
 
 def partials_bytes(value_type: int, accumulate: int, n_chunks: int, n_dense: int = 1) -> int:
     return int(load().fs_partials_bytes(value_type, accumulate, n_chunks, n_dense))
+
+
+def sliced_combine(sl: FsSlices, z, y, alpha: float, beta: float, device) -> None:
+    check(load().fs_sliced_combine(ctypes.byref(sl), ptr(z), ptr(y), float(alpha), float(beta),
+                                   stream_handle(device)), "fs_sliced_combine")
+
+
+def tile_carry_bytes(value_type: int, n_tiles: int) -> int:
+    return int(load().fs_tile_carry_bytes(value_type, n_tiles))
 
 
 def version() -> str:

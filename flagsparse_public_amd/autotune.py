@@ -23,31 +23,41 @@ import torch
 
 from .rowsplit import Schedule
 
-VARIANTS_VERSION = 1
+VARIANTS_VERSION = 2
 
-# SpMV (fp32, native accumulation) variants: group width, short-row bound,
-# chunk size, unroll, XCD remap.  Index 0..6 sweep the group width with the
-# default rule's bounds; the rest vary depth, chunking and locality.
+# SpMV (fp32, native accumulation) variants.  0-5: entry tiles (fixed runs of
+# entries), 3-5 with the columns cut into L2-sized slices when x is larger
+# than one slice; 6-8: row tiles with chunked long rows; 9-15: one lane group
+# per row (S lanes), long rows chunked.
 SPMV_VARIANTS = (
+    Schedule(tile_nnz=1024, tile_mode=1),
+    Schedule(tile_nnz=2048, tile_mode=1),
+    Schedule(tile_nnz=4096, tile_mode=1),
+    Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=512 * 1024),
+    Schedule(tile_nnz=4096, tile_mode=1, x_slice_cols=256 * 1024),
+    Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=256 * 1024),
+    Schedule(group_lanes=8, short_max=2048, chunk_nnz=8192, unroll=4, tile_nnz=2048),
+    Schedule(group_lanes=8, short_max=1024, chunk_nnz=4096, unroll=8, tile_nnz=1024),
+    Schedule(group_lanes=8, short_max=256, chunk_nnz=2048, unroll=4, tile_nnz=512),
     Schedule(group_lanes=1, short_max=64, chunk_nnz=512, unroll=4),
     Schedule(group_lanes=2, short_max=64, chunk_nnz=512, unroll=4),
-    Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=4),
+    Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=8),
     Schedule(group_lanes=8, short_max=128, chunk_nnz=512, unroll=4),
     Schedule(group_lanes=16, short_max=256, chunk_nnz=512, unroll=4),
     Schedule(group_lanes=32, short_max=512, chunk_nnz=1024, unroll=4),
-    Schedule(group_lanes=64, short_max=1024, chunk_nnz=2048, unroll=4),
-    Schedule(group_lanes=2, short_max=32, chunk_nnz=256, unroll=2),
-    Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=8),
-    Schedule(group_lanes=8, short_max=128, chunk_nnz=512, unroll=8),
-    Schedule(group_lanes=8, short_max=64, chunk_nnz=256, unroll=4),
-    Schedule(group_lanes=16, short_max=1024, chunk_nnz=2048, unroll=8),
-    Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=4, xcd_remap=1),
-    Schedule(group_lanes=8, short_max=128, chunk_nnz=512, unroll=4, xcd_remap=1),
-    Schedule(group_lanes=16, short_max=256, chunk_nnz=512, unroll=4, xcd_remap=1),
-    Schedule(group_lanes=32, short_max=4096, chunk_nnz=4096, unroll=8, xcd_remap=1),
+    Schedule(group_lanes=64, short_max=4096, chunk_nnz=4096, unroll=8),
 )
-# The fixed "plain default-tile" SpMV schedule the selector is compared with.
-SPMV_DEFAULT_INDEX = 4
+# The plain default-tile SpMV the learned selector is compared with: one
+# 8-lane group per row with fixed bounds (the analogue of the reference's
+# default-path kernel with its fixed 256-entry tile, spmv_csr.py:158-184).
+SPMV_DEFAULT_INDEX = 12
+
+
+def spmv_heuristic_index(n_cols: int) -> int:
+    """Variant equal to the fixed rule of rowsplit.default_spmv_schedule for
+    fp32 (entry tiles of 2048; 2 MB column slices once x exceeds 4 MB)."""
+    return 3 if n_cols * 4 > (4 << 20) else 1
+
 
 # SpMM (64 dense columns) variants: column tile (lane-group width), short-row
 # bound, chunk size, XCD remap.

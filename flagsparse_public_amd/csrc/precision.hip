@@ -24,6 +24,7 @@ struct SplitBf16Vals {
   template <typename C> __device__ __forceinline__ C get(int64_t j) const {
     return (C)(bf16_to_f32(hi[j]) + bf16_to_f32(lo[j]));
   }
+  static constexpr bool kVec = false;
   static constexpr bool kRowScale = false;
   __device__ __forceinline__ float row_scale(int64_t) const { return 1.f; }
 };
@@ -31,6 +32,7 @@ struct SplitBf16Vals {
 struct HiBf16Vals {
   const uint16_t* hi;
   template <typename C> __device__ __forceinline__ C get(int64_t j) const { return (C)bf16_to_f32(hi[j]); }
+  static constexpr bool kVec = false;
   static constexpr bool kRowScale = false;
   __device__ __forceinline__ float row_scale(int64_t) const { return 1.f; }
 };
@@ -39,6 +41,7 @@ struct ScaledF16Vals {
   const uint16_t* h;
   const float* scale;
   template <typename C> __device__ __forceinline__ C get(int64_t j) const { return (C)f16_to_f32(h[j]); }
+  static constexpr bool kVec = false;
   static constexpr bool kRowScale = true;
   __device__ __forceinline__ float row_scale(int64_t r) const { return scale[r]; }
 };
@@ -101,6 +104,8 @@ int launch_split_s(const fs_csr* M, const VS& vs, const void* x, void* y, double
   if (grid == 0) return FS_OK;
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr_split: grid of %lld blocks too large", (long long)grid);
+  if (pl->tile_nnz > 0 && pl->tile_mode == FS_TILE_ENTRIES)
+    return spmv_launch_entry<Plain<float>, false>(p, grid, st, "fs_spmv_csr_split");
   if (pl->tile_nnz > 0) {
     if (tile_lds_bytes<double>(pl->tile_nnz) > 65536)
       return fail(FS_ERR_VALUE, "fs_spmv_csr_split: tile_nnz %lld needs more than 64 KB of LDS",

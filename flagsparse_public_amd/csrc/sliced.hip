@@ -1,0 +1,104 @@
+// sliced.hip -- the combine pass of the column-sliced SpMV.
+//
+// A matrix whose x does not fit one XCD's L2 is stored column-slice-major:
+// the entries of slice k (columns [k*w, (k+1)*w)) come before those of slice
+// k + 1, grouped by row inside a slice.  Each (slice, row) pair with entries
+// is a "compact row"; an ordinary SpMV over the compact rows (entry tiles,
+// block groups pinned to XCDs so each XCD's L2 holds only its own x columns)
+// writes one partial z[i] per compact row.  This pass then forms
+//   y[r] = alpha * sum_k z[compact row (k, r)] + beta * y[r]
+// with the sum taken in slice order in fp64 (complex: two fp64), so the
+// result does not depend on scheduling.
+//
+// No counterpart in the reference (its SpMV reads x in place,
+// spmv_csr.py:158-184); the row partition and the error contract are those of
+// fs_spmv_csr.
+#include "spmv_kernel.hpp"
+
+namespace fs {
+
+constexpr int kCombineRows = 256;  // rows per block (one per thread)
+
+// bounds[k * (n_blocks + 1) + b]: first compact row of slice k whose row is
+// in row block b (rows [b * 256, (b + 1) * 256)).  A row appears at most once
+// per slice, so a (slice, block) range holds at most 256 compact rows: thread
+// t takes the t-th of each.  The loads of kGroup slices are issued together,
+// then added to the LDS row sums one slice at a time.
+constexpr int kGroup = 8;
+
+template <typename V>
+__global__ __launch_bounds__(kCombineRows) void sliced_combine_kernel(
+    const V* __restrict__ z, const int32_t* __restrict__ compact_row,
+    const int64_t* __restrict__ bounds, int32_t n_slices, int64_t n_rows, int64_t n_blocks,
+    V* __restrict__ y, double alpha, double beta) {
+  using C = typename Native<V>::T;
+  using S = typename TileSum<C>::T;
+  __shared__ S acc[kCombineRows];
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t row0 = b * kCombineRows;
+  acc[tid] = zero<S>();
+  for (int k0 = 0; k0 < n_slices; k0 += kGroup) {
+    C zv[kGroup];
+    int rv[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const int k = k0 + u;
+      rv[u] = -1;
+      zv[u] = zero<C>();
+      if (k < n_slices) {
+        const int64_t lo = bounds[(int64_t)k * (n_blocks + 1) + b];
+        const int64_t hi = bounds[(int64_t)k * (n_blocks + 1) + b + 1];
+        const int64_t i = lo + tid;
+        if (i < hi) {
+          zv[u] = load_as<C>(z, i);
+          const int64_t r = (int64_t)compact_row[i] - row0;
+          if (r >= 0 && r < kCombineRows) rv[u] = (int)r;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      __syncthreads();  // slice order: rows are distinct inside a slice
+      if (rv[u] >= 0) acc[rv[u]] = add(acc[rv[u]], widen(zv[u]));
+    }
+  }
+  __syncthreads();
+  const int64_t row = row0 + tid;
+  if (row < n_rows) write_out(y, row, acc[tid], alpha, beta);
+}
+
+template <typename V>
+int launch_combine(const fs_slices* sl, const void* z, void* y, double alpha, double beta,
+                   hipStream_t st) {
+  const int64_t nb = (sl->n_rows + kCombineRows - 1) / kCombineRows;
+  if (nb == 0) return FS_OK;
+  sliced_combine_kernel<V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
+      static_cast<const V*>(z), sl->compact_row, sl->bounds, sl->n_slices, sl->n_rows, nb,
+      static_cast<V*>(y), alpha, beta);
+  return check_launch("fs_sliced_combine");
+}
+
+}  // namespace fs
+
+using namespace fs;
+
+extern "C" int fs_sliced_combine(const fs_slices* sl, const void* z, void* y, double alpha,
+                                 double beta, void* stream) {
+  if (!sl) return fail(FS_ERR_VALUE, "fs_sliced_combine: slices descriptor is null");
+  if (sl->n_rows < 0 || sl->n_compact < 0 || sl->n_slices <= 0)
+    return fail(FS_ERR_VALUE, "fs_sliced_combine: bad sizes");
+  if (sl->rows_per_block != kCombineRows)
+    return fail(FS_ERR_VALUE, "fs_sliced_combine: rows_per_block must be %d", kCombineRows);
+  if (sl->n_rows == 0) return FS_OK;
+  if (!y || !sl->bounds || (sl->n_compact > 0 && (!z || !sl->compact_row)))
+    return fail(FS_ERR_VALUE, "fs_sliced_combine: null pointer");
+  hipStream_t st = as_stream(stream);
+  switch (sl->value_type) {
+    case FS_F32: return launch_combine<float>(sl, z, y, alpha, beta, st);
+    case FS_F64: return launch_combine<double>(sl, z, y, alpha, beta, st);
+    case FS_C64: return launch_combine<c64_t>(sl, z, y, alpha, beta, st);
+    case FS_C128: return launch_combine<c128_t>(sl, z, y, alpha, beta, st);
+    default: return fail(FS_ERR_TYPE, "fs_sliced_combine: value_type %d is not sliced", sl->value_type);
+  }
+}

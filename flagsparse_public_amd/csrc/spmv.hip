@@ -64,8 +64,28 @@ int launch_spmv_s(const fs_csr* M, const void* x, void* y, double alpha, double 
 }
 
 template <typename V, typename A, typename I, typename P>
+int launch_spmv_entries(const fs_csr* M, const void* x, void* y, double alpha, double beta,
+                        const fs_plan* pl, hipStream_t st) {
+  using VS = DirectVals<V>;
+  SpmvParams<VS, V, I, P> p;
+  p.ptr = static_cast<const P*>(M->row_ptr);
+  p.idx = static_cast<const I*>(M->col_idx);
+  p.vals.v = static_cast<const V*>(M->values);
+  p.x = static_cast<const V*>(x);
+  p.y = static_cast<V*>(y);
+  p.alpha = alpha;
+  p.beta = beta;
+  const int64_t grid = spmv_fill_schedule(p, M, pl, 1);
+  if (grid > 0x7fffffffLL)
+    return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr: grid of %lld blocks too large", (long long)grid);
+  return spmv_launch_entry<A, kHotSpmv<V, I>>(p, grid, st, "fs_spmv_csr");
+}
+
+template <typename V, typename A, typename I, typename P>
 int launch_spmv(const fs_csr* M, const void* x, void* y, double alpha, double beta,
                 const fs_plan* pl, hipStream_t st) {
+  if (pl->tile_nnz > 0 && pl->tile_mode == FS_TILE_ENTRIES)
+    return launch_spmv_entries<V, A, I, P>(M, x, y, alpha, beta, pl, st);
   const int s = pl->group_lanes;
   if constexpr (kHotSpmv<V, I>) {
     switch (s) {

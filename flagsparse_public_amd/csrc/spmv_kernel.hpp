@@ -25,11 +25,37 @@ namespace fs {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 
+// Four consecutive elements p[j .. j+3] with one 8-byte or one to four 16-byte
+// loads; needs j % 4 == 0 and p aligned to min(16, 4 * sizeof(T)) bytes.
+template <typename T>
+__device__ __forceinline__ void load4(const T* __restrict__ p, int64_t j, T (&o)[4]) {
+  static_assert(sizeof(T) == 2 || sizeof(T) % 4 == 0, "element size");
+  if constexpr (sizeof(T) == 2) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p + j);
+    __builtin_memcpy(o, &v, 8);
+  } else {
+    constexpr int N = (int)sizeof(T) * 4 / 16;
+    const uint4* q = reinterpret_cast<const uint4*>(p + j);
+    uint4 v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = q[k];
+    __builtin_memcpy(o, v, sizeof(T) * 4);
+  }
+}
+
 // Value source of plainly stored values of type V.
 template <typename V>
 struct DirectVals {
   const V* v;
   template <typename C> __device__ __forceinline__ C get(int64_t j) const { return load_as<C>(v, j); }
+  // vector form for the entry-tile role (j % 4 == 0, v aligned)
+  static constexpr bool kVec = true;
+  template <typename C> __device__ __forceinline__ void get4(int64_t j, C (&o)[4]) const {
+    V t[4];
+    load4(v, j, t);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = load_as<C>(t, k);
+  }
   static constexpr bool kRowScale = false;
   __device__ __forceinline__ float row_scale(int64_t) const { return 1.f; }
 };
@@ -55,6 +81,9 @@ struct SpmvParams {
   int64_t tile_nnz;            // tile role: target entries per tile (0: row-group role)
   int64_t n_tiles;
   const int64_t* tile_start;   // [n_tiles + 1] first row of each tile
+  int64_t nnz;
+  int n_groups;                // entry tiles: block groups (0: block b = tile b)
+  const int64_t* group_tiles;  // [n_groups + 1]
 };
 
 // Accumulator used for chunks: fp32-native chunked rows get compensation.
@@ -305,6 +334,13 @@ int64_t spmv_fill_schedule(SpmvParams<VS, V, I, P>& p, const fs_csr* M, const fs
   p.tile_nnz = pl->tile_nnz;
   p.n_tiles = pl->n_tiles;
   p.tile_start = pl->tile_start;
+  p.nnz = M->nnz;
+  p.n_groups = pl->n_groups;
+  p.group_tiles = pl->group_tiles;
+  if (pl->tile_nnz > 0 && pl->tile_mode == FS_TILE_ENTRIES) {
+    if (pl->n_groups <= 0) return pl->n_tiles;
+    return pl->group_blocks * pl->n_groups;
+  }
   if (pl->tile_nnz > 0) return p.n_chunk_blocks + pl->n_tiles;
   const int64_t rows_per_block = kBlock / S;
   return p.n_chunk_blocks + (M->n_rows + rows_per_block - 1) / rows_per_block;
@@ -315,6 +351,254 @@ template <typename A, int U, typename VS, typename V, typename I, typename P>
 void spmv_launch_tiles(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st) {
   using S = typename TileSum<typename A::compute_t>::T;
   spmv_tile_kernel<A, U><<<dim3((unsigned)grid), dim3(kBlock), tile_lds_bytes<S>(p.tile_nnz), st>>>(p);
+}
+
+// ------------------------------------------------------------ entry tiles
+//
+// Tile t holds the entries [t*T, min((t+1)*T, nnz)) (T = tile_nnz = E * kBlock)
+// and owns the rows r with t*T <= row_ptr[r] < (t+1)*T (the last tile also
+// owns the rows that start at nnz); tile_start[t] is its first owned row.  The
+// entry range of a tile is known at launch, so every thread issues all E of
+// its col / value loads (16-byte vector loads when VEC) before the first x
+// gather, and the first batch of row pointers is loaded beside them.  Products
+// go to LDS in entry order; every owned row is summed there in fp64 (complex:
+// two fp64) in entry order -- one thread per row up to kThreadRowMax entries,
+// one wavefront above.  Two pieces can run past a tile: the head (entries of
+// the last row of an earlier tile) and the tile's own last row; their sums go
+// to carries[2t] and carries[2t+1], and spmv_entry_fixup_kernel adds a
+// straddling row's carries in tile order.  Rows of any length take the same
+// path; results depend on T only through where a long row is cut.
+
+constexpr int kSegRows = 1024;  // row boundaries staged in LDS per pass
+
+template <typename C> __host__ __device__ inline size_t entry_lds_bytes(int64_t tile_nnz) {
+  using S = typename TileSum<C>::T;
+  return sizeof(C) * (size_t)tile_nnz + sizeof(int32_t) * (kSegRows + 4) +
+         sizeof(S) * kWavesPerBlock;
+}
+
+template <typename A, typename VS, typename V, typename I, typename P, typename S>
+__device__ __forceinline__ void entry_finish(const SpmvParams<VS, V, I, P>& p, int64_t row,
+                                             S acc, bool to_carry, S* carries, int64_t t) {
+  if (to_carry) carries[2 * t + 1] = acc;
+  else write_row(p.vals, p.y, row, acc, p.alpha, p.beta);
+}
+
+template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>
+__global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I, P> p) {
+  using C = typename A::compute_t;
+  using S = typename TileSum<C>::T;
+  constexpr int T = E * kBlock;
+  constexpr int kPv = kSegRows / kBlock + 1;
+  extern __shared__ __align__(16) char smem[];
+  C* prod = reinterpret_cast<C*>(smem);
+  int32_t* bnd = reinterpret_cast<int32_t*>(smem + sizeof(C) * T);
+  S* red = reinterpret_cast<S*>(smem + sizeof(C) * T + sizeof(int32_t) * (kSegRows + 4));
+  S* carries = reinterpret_cast<S*>(p.partials);
+  const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+  int64_t t = blockIdx.x;
+  if (p.n_groups > 0) {  // block b serves group b % G, in order within the group
+    const int g = (int)(t % p.n_groups);
+    t = p.group_tiles[g] + t / p.n_groups;
+    if (t >= p.group_tiles[g + 1] || t >= p.n_tiles) return;
+  }
+  const int64_t e0 = t * T;
+  const int64_t rem = p.nnz - e0;
+  const int len = rem < T ? (int)(rem > 0 ? rem : 0) : T;
+  // owned rows [r0, r1), clamped to [0, n_rows] so a damaged plan cannot
+  // send a row-pointer read out of bounds
+  int64_t r0 = p.tile_start[t], r1 = p.tile_start[t + 1];
+  r0 = r0 < 0 ? 0 : (r0 > p.n_rows ? p.n_rows : r0);
+  r1 = r1 < r0 ? r0 : (r1 > p.n_rows ? p.n_rows : r1);
+  const int nb0 = (int)(r1 - r0 < kSegRows ? r1 - r0 : kSegRows);
+  // first batch of row pointers (i <= nb0), loaded beside the entries
+  int64_t pv[kPv];
+#pragma unroll
+  for (int k = 0; k < kPv; ++k) {
+    const int i = k * kBlock + tid;
+    pv[k] = i <= nb0 ? (int64_t)p.ptr[r0 + i] : 0;
+  }
+  if (len == T) {
+    I col[E];
+    C a[E];
+    if constexpr (VEC) {
+#pragma unroll
+      for (int g = 0; g < E / 4; ++g) {
+        const int64_t j = e0 + (int64_t)(g * kBlock + tid) * 4;
+        I ci[4];
+        load4(p.idx, j, ci);
+        C av[4];
+        p.vals.template get4<C>(j, av);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          col[4 * g + k] = ci[k];
+          a[4 * g + k] = av[k];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const int64_t j = e0 + u * kBlock + tid;
+        col[u] = p.idx[j];
+        a[u] = p.vals.template get<C>(j);
+      }
+    }
+    C xv[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) xv[u] = load_as<C>(p.x, (int64_t)col[u]);
+    if constexpr (VEC) {
+#pragma unroll
+      for (int g = 0; g < E / 4; ++g)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) prod[(g * kBlock + tid) * 4 + k] = mulc(a[4 * g + k], xv[4 * g + k]);
+    } else {
+#pragma unroll
+      for (int u = 0; u < E; ++u) prod[u * kBlock + tid] = mulc(a[u], xv[u]);
+    }
+  } else {
+    // the last, partial tile: scalar loads with bounds
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int o = u * kBlock + tid;
+      if (o < len) {
+        const int64_t j = e0 + o;
+        prod[o] = mulc(p.vals.template get<C>(j), load_as<C>(p.x, ldg_index(p.idx, j)));
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kPv; ++k) {
+    const int i = k * kBlock + tid;
+    if (i <= nb0) {
+      const int64_t v = pv[k] - e0;
+      bnd[i] = (int)(v < 0 ? 0 : (v > len ? len : v));
+    }
+  }
+  const bool straddle = r1 > r0 && (int64_t)p.ptr[r1] > e0 + len;
+  __syncthreads();
+  // head: entries [0, bnd[0]) belong to a row owned by an earlier tile
+  const int head = bnd[0];
+  if (head > 0 && head <= 4 * kWave) {
+    if (w == 0) {
+      Plain<S> acc;
+      acc.init();
+      for (int j = lane; j < head; j += kWave) acc.add_value(widen(prod[j]));
+      group_reduce<kWave>(acc);
+      if (lane == 0) carries[2 * t] = acc.value();
+    }
+  } else if (head > 0) {
+    Plain<S> acc;
+    acc.init();
+    for (int j = tid; j < head; j += kBlock) acc.add_value(widen(prod[j]));
+    group_reduce<kWave>(acc);
+    if (lane == 0) red[w] = acc.value();
+    __syncthreads();
+    if (tid == 0) {
+      S tot = red[0];
+      for (int k = 1; k < kWavesPerBlock; ++k) tot = add(tot, red[k]);
+      carries[2 * t] = tot;
+    }
+  } else if (tid == 0) {
+    carries[2 * t] = zero<S>();
+  }
+  for (int64_t rb = r0; rb < r1; rb += kSegRows) {
+    const int nb = (int)(r1 - rb < kSegRows ? r1 - rb : kSegRows);
+    if (rb != r0) {  // later batches (tiles owning more than kSegRows rows)
+      __syncthreads();
+      for (int i = tid; i <= nb; i += kBlock) {
+        const int64_t v = (int64_t)p.ptr[rb + i] - e0;
+        bnd[i] = (int)(v < 0 ? 0 : (v > len ? len : v));
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < nb; i += kBlock) {
+      const int s = bnd[i], e = bnd[i + 1];
+      if (e - s > kThreadRowMax) continue;
+      S acc = zero<S>();
+      for (int j = s; j < e; ++j) acc = add(acc, widen(prod[j]));
+      const int64_t row = rb + i;
+      entry_finish<A>(p, row, acc, straddle && row == r1 - 1, carries, t);
+    }
+    for (int i = w; i < nb; i += kWavesPerBlock) {
+      const int s = bnd[i], e = bnd[i + 1];
+      if (e - s <= kThreadRowMax) continue;
+      Plain<S> acc;
+      acc.init();
+      for (int j = s + lane; j < e; j += kWave) acc.add_value(widen(prod[j]));
+      group_reduce<kWave>(acc);
+      const int64_t row = rb + i;
+      if (lane == 0) entry_finish<A>(p, row, acc.value(), straddle && row == r1 - 1, carries, t);
+    }
+  }
+}
+
+// Rows cut by a tile boundary: thread t takes tile t's last row when it runs
+// past the tile and adds the heads of the following tiles in order until the
+// tile that owns the next row (or the last tile).
+template <typename A, typename VS, typename V, typename I, typename P>
+__global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS, V, I, P> p) {
+  using S = typename TileSum<typename A::compute_t>::T;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t + 1 >= p.n_tiles) return;  // the last tile ends at nnz: no row runs past it
+  const int64_t r0 = p.tile_start[t], r1 = p.tile_start[t + 1];
+  if (r1 <= r0 || r1 > p.n_rows) return;
+  if ((int64_t)p.ptr[r1] <= (t + 1) * p.tile_nnz) return;
+  const S* carries = reinterpret_cast<const S*>(p.partials);
+  S acc = carries[2 * t + 1];
+  for (int64_t u = t + 1; u < p.n_tiles; ++u) {
+    acc = add(acc, carries[2 * u]);
+    if (p.tile_start[u + 1] > p.tile_start[u]) break;
+  }
+  write_row(p.vals, p.y, r1 - 1, acc, p.alpha, p.beta);
+}
+
+template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>
+void spmv_launch_entry_e(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st) {
+  spmv_entry_kernel<A, E, VEC><<<dim3((unsigned)grid), dim3(kBlock),
+                                 entry_lds_bytes<typename A::compute_t>(E * kBlock), st>>>(p);
+}
+
+// Whether the col / value arrays allow the 16-byte loads of the entry role.
+template <typename VS, typename V, typename I, typename P>
+bool entry_vec_ok(const SpmvParams<VS, V, I, P>& p) {
+  if constexpr (!VS::kVec) {
+    return false;
+  } else {
+    const size_t va = 4 * sizeof(V) >= 16 ? 16 : 4 * sizeof(V);
+    return ((uintptr_t)p.idx % 16) == 0 && ((uintptr_t)p.vals.v % va) == 0;
+  }
+}
+
+// Launches the entry-tile kernel and its fix-up.  HOT builds tiles of 1024,
+// 2048 and 4096 entries with and without vector loads; others 2048, scalar.
+template <typename A, bool HOT, typename VS, typename V, typename I, typename P>
+int spmv_launch_entry(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st,
+                      const char* who) {
+  using C = typename A::compute_t;
+  if (entry_lds_bytes<C>(p.tile_nnz) > 65536)
+    return fail(FS_ERR_VALUE, "%s: entry tiles of %lld entries need more than 64 KB of LDS", who,
+                (long long)p.tile_nnz);
+  const bool vec = entry_vec_ok(p);
+  bool done = false;
+  if constexpr (HOT) {
+    switch (p.tile_nnz) {
+      case 1024: vec ? spmv_launch_entry_e<A, 4, true>(p, grid, st) : spmv_launch_entry_e<A, 4, false>(p, grid, st); done = true; break;
+      case 2048: vec ? spmv_launch_entry_e<A, 8, true>(p, grid, st) : spmv_launch_entry_e<A, 8, false>(p, grid, st); done = true; break;
+      case 4096: vec ? spmv_launch_entry_e<A, 16, true>(p, grid, st) : spmv_launch_entry_e<A, 16, false>(p, grid, st); done = true; break;
+      default: break;
+    }
+  } else if (p.tile_nnz == 2048) {
+    spmv_launch_entry_e<A, 8, false>(p, grid, st);
+    done = true;
+  }
+  if (!done)
+    return fail(FS_ERR_VALUE, "%s: entry tiles of %lld entries are not built for this type", who,
+                (long long)p.tile_nnz);
+  int rc = check_launch(who);
+  if (rc || p.n_tiles < 2) return rc;
+  const int64_t blocks = (p.n_tiles + kBlock - 1) / kBlock;
+  spmv_entry_fixup_kernel<A><<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(p);
+  return check_launch(who);
 }
 
 // Launches the fix-up pass when some long row spans several chunks.

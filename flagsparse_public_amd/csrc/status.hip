@@ -67,6 +67,24 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
     return fail(FS_ERR_VALUE, "%s: chunks listed without long rows", who);
   }
   if (P->tile_nnz < 0 || P->n_tiles < 0) return fail(FS_ERR_VALUE, "%s: negative tile fields", who);
+  if (P->tile_mode != FS_TILE_ROWS && P->tile_mode != FS_TILE_ENTRIES)
+    return fail(FS_ERR_VALUE, "%s: tile_mode must be FS_TILE_ROWS or FS_TILE_ENTRIES", who);
+  if (P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES) {
+    if (n_dense != 1) return fail(FS_ERR_VALUE, "%s: entry tiles are for SpMV", who);
+    if (P->n_long != 0) return fail(FS_ERR_VALUE, "%s: entry tiles take no long-row lists", who);
+    const int64_t want = A->nnz > 0 ? (A->nnz + P->tile_nnz - 1) / P->tile_nnz : 1;
+    if (P->n_tiles != want)
+      return fail(FS_ERR_VALUE, "%s: entry tiles need n_tiles = %lld, got %lld", who,
+                  (long long)want, (long long)P->n_tiles);
+    if (!P->tile_start) return fail(FS_ERR_VALUE, "%s: entry tiles need tile_start", who);
+    if (P->n_groups < 0 || (P->n_groups > 0 && (!P->group_tiles || P->group_blocks < 0)))
+      return fail(FS_ERR_VALUE, "%s: bad block groups", who);
+    const int64_t need = fs_tile_carry_bytes(A->value_type, P->n_tiles);
+    if (!P->partials || P->partials_bytes < need)
+      return fail(FS_ERR_VALUE, "%s: carry buffer too small (%lld < %lld bytes)", who,
+                  (long long)P->partials_bytes, (long long)need);
+    return FS_OK;
+  }
   if (P->tile_nnz > 0) {
     if (n_dense != 1) return fail(FS_ERR_VALUE, "%s: the tile role is for SpMV", who);
     if (P->short_max > P->tile_nnz)
@@ -88,4 +106,10 @@ extern "C" int64_t fs_partials_bytes(int32_t value_type, int32_t accumulate, int
   const int64_t e = fs::partial_elem_bytes(value_type, accumulate);
   if (e < 0 || n_chunks < 0) return -1;
   return e * n_chunks * (n_dense > 0 ? n_dense : 1);
+}
+
+extern "C" int64_t fs_tile_carry_bytes(int32_t value_type, int64_t n_tiles) {
+  if (n_tiles < 0 || value_type < FS_F16 || value_type > FS_C128) return -1;
+  const bool cplx = value_type == FS_C32 || value_type == FS_C64 || value_type == FS_C128;
+  return 2 * n_tiles * (cplx ? 16 : 8);
 }

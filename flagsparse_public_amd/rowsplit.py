@@ -33,6 +33,7 @@ class Schedule:
     tile_nnz: int = 0         # SpMV: > 0 selects the tile role for short rows
     x_slice_cols: int = 0     # SpMV native path: > 0 cuts the columns into slices of
                               # at most this many (one launch per slice, x slice in L2)
+    tile_mode: int = 0        # SpMV with tile_nnz > 0: 0 row tiles, 1 entry tiles
 
     def key(self) -> tuple:
         return dataclasses.astuple(self)
@@ -48,10 +49,27 @@ def _pow2_ceil(v: float) -> int:
 TILE_ROWS = 512  # rows per tile at most (the kernel's LDS row-pointer slots)
 
 
-def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None) -> Schedule:
-    """Fixed rule used when no selector is supplied: tiles of ~2048 entries for
-    rows up to 2048 entries (1024 for complex values, whose fp64 row sums take
-    16 bytes of LDS each), longer rows in 8192-entry chunks."""
+L2_SLICE_BYTES = 2 << 20   # x columns per XCD slice: half of one XCD's 4 MB L2
+SLICE_MIN_X_BYTES = 4 << 20  # slice only when x is larger than one L2
+
+
+def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None, n_cols: int | None = None
+                          ) -> Schedule:
+    """Fixed rule used when no selector is supplied: entry tiles of 2048
+    entries; when x is larger than one XCD's L2 (fp32 / fp64 / complex values),
+    the native path also cuts the columns into slices of 2 MB of x."""
+    slice_cols = 0
+    if value_dtype is not None and n_cols is not None:
+        item = torch.empty(0, dtype=value_dtype).element_size()
+        if n_cols * item > SLICE_MIN_X_BYTES:
+            slice_cols = L2_SLICE_BYTES // item
+    return Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=slice_cols)
+
+
+def row_tile_spmv_schedule(n_rows: int, nnz: int, value_dtype=None) -> Schedule:
+    """Row tiles of ~2048 entries for rows up to 2048 entries (1024 for
+    complex values, whose fp64 row sums take 16 bytes of LDS each), longer
+    rows in 8192-entry chunks."""
     t = 1024 if (value_dtype is not None and value_dtype.is_complex) else 2048
     return Schedule(group_lanes=8, short_max=t, chunk_nnz=8192, unroll=4, tile_nnz=t)
 
@@ -110,6 +128,12 @@ class RowSplitPlan:
         self.accumulate = int(accumulate)
         self.value_code = _lib.value_code(value_dtype)
         self.device = device
+        self._work = {}
+        self._structs = {}
+        self.n_groups, self.group_blocks, self.group_tiles = 0, 0, None
+        if schedule.tile_nnz > 0 and schedule.tile_mode == _lib.FS_TILE_ENTRIES:
+            self._init_entry_tiles(indptr, n_rows, schedule)
+            return
         if row_lengths is None:
             row_lengths = (indptr[1:].to(torch.int64) - indptr[:-1].to(torch.int64)) if n_rows > 0 \
                 else torch.zeros(0, dtype=torch.int64, device=device)
@@ -136,14 +160,62 @@ class RowSplitPlan:
         else:
             self.tile_start = None
             self.n_tiles = 0
-        self._work = {}  # n_dense -> partials
-        self._structs = {}
+
+    def _init_entry_tiles(self, indptr, n_rows, schedule):
+        """Entry tiles: tile t holds entries [t*T, (t+1)*T) and owns the rows
+        whose first entry falls in it (the last tile also the rows that start
+        at nnz)."""
+        device = indptr.device
+        T = int(schedule.tile_nnz)
+        nnz = int(indptr[-1].item()) if n_rows > 0 else 0
+        self.n_tiles = max(1, -(-nnz // T))
+        starts = torch.zeros(self.n_tiles + 1, dtype=torch.int64, device=device)
+        starts[-1] = n_rows
+        if self.n_tiles > 1:
+            bounds = torch.arange(1, self.n_tiles, dtype=torch.int64, device=device) * T
+            starts[1:-1] = torch.searchsorted(indptr[:-1].to(torch.int64).contiguous(), bounds,
+                                              side="left")
+        self.tile_start = starts
+        self.long_rows = torch.zeros(0, dtype=torch.int64, device=device)
+        self.long_chunk_ptr = torch.zeros(1, dtype=torch.int64, device=device)
+        self.chunk_long = torch.zeros(0, dtype=torch.int32, device=device)
+        self.n_long = 0
+        self.n_chunks = 0
+        self.entry_tiles = True
+
+    def set_groups(self, n_groups: int, group_blocks: int, group_tiles: torch.Tensor):
+        """Entry tiles: deal tile ranges to block groups (block b serves group
+        b % n_groups); group_tiles[g] .. group_tiles[g + 1] must cover every
+        tile once and group_blocks be the largest range."""
+        self.n_groups = int(n_groups)
+        self.group_blocks = int(group_blocks)
+        self.group_tiles = group_tiles
+        self._structs.clear()
 
     def struct(self, n_dense: int = 1) -> _lib.FsPlan:
         """fs_plan for a call with n_dense dense columns (1 for SpMV)."""
         key = max(int(n_dense), 1)
         st = self._structs.get(key)
         if st is not None:
+            return st
+        if getattr(self, "entry_tiles", False):
+            if key != 1:
+                raise ValueError("entry tiles are an SpMV schedule")
+            nbytes = _lib.tile_carry_bytes(self.value_code, self.n_tiles)
+            carries = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=self.device)
+            self._work[key] = carries
+            sch = self.schedule
+            st = _lib.FsPlan(accumulate=self.accumulate, group_lanes=sch.group_lanes,
+                             xcd_remap=0, unroll=sch.unroll, short_max=0, chunk_nnz=64,
+                             n_long=0, long_rows=_lib.ptr(self.long_rows),
+                             long_chunk_ptr=_lib.ptr(self.long_chunk_ptr), n_chunks=0,
+                             chunk_long=_lib.ptr(self.chunk_long), partials=carries.data_ptr(),
+                             partials_bytes=int(carries.numel()), tile_nnz=sch.tile_nnz,
+                             n_tiles=self.n_tiles, tile_start=_lib.ptr(self.tile_start),
+                             tile_mode=_lib.FS_TILE_ENTRIES, n_groups=self.n_groups,
+                             group_blocks=self.group_blocks,
+                             group_tiles=_lib.ptr(self.group_tiles))
+            self._structs[key] = st
             return st
         if self.n_long:
             nbytes = _lib.partials_bytes(self.value_code, self.accumulate, self.n_chunks, key)
@@ -169,6 +241,7 @@ class RowSplitPlan:
             tile_nnz=sch.tile_nnz if key == 1 else 0,
             n_tiles=self.n_tiles if key == 1 else 0,
             tile_start=_lib.ptr(self.tile_start) if (key == 1 and self.tile_start is not None) else None,
+            tile_mode=_lib.FS_TILE_ROWS,
         )
         self._structs[key] = st
         return st

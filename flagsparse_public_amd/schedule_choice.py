@@ -28,7 +28,7 @@ def choose_schedule(op, request, data, n_rows, n_cols, stats_dev):
     mode = "learned" if request is None else str(request)
     if mode not in _MODES:
         raise ValueError(f"schedule must be a Schedule or one of {_MODES}")
-    fallback = (default_spmv_schedule(n_rows, nnz, data.dtype) if op == "spmv"
+    fallback = (default_spmv_schedule(n_rows, nnz, data.dtype, n_cols) if op == "spmv"
                 else default_spmm_schedule(n_rows, nnz, 64))
     if mode == "default" or n_rows == 0:
         return fallback, "default"

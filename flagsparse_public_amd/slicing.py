@@ -1,19 +1,22 @@
-"""Column-sliced SpMV: keep the x gathers inside L2.
+"""Column-sliced SpMV: each XCD's L2 holds only its own x columns.
 
-A matrix whose x does not fit an XCD's L2 (4 MB on MI355X) pays one L2 miss
--- a full cache-line fill from the Infinity Cache -- for most of its 4-byte
-x gathers.  Cutting the columns into P slices of at most ``slice_cols``
-columns and running one launch per slice keeps every launch's x slice
-resident in every XCD's L2: launch k computes y = alpha A_k x_k + beta' y with
-beta' = beta for k = 0 and 1 afterwards.  The cost is P row-pointer reads and
-P - 1 extra read-modify-writes of y, paid in streaming bytes instead of
-fabric round trips.
+A random x gather that misses L2 costs a full cache-line fill from the
+Infinity Cache; with x far larger than one XCD's 4 MB L2, those fills, not
+HBM, bound the SpMV.  Here the columns are cut into P = 8q slices of at most
+``slice_cols`` columns and the matrix is stored slice-major: the entries of
+slice k come before those of slice k + 1, grouped by row inside a slice.
+Every (slice, row) pair with entries is a *compact row*.
 
-The per-slice matrices are built once (stable sort of the entries by
-(slice, row), column indices made slice-relative), so entries of a row keep
-their order inside each slice and the result is deterministic.  Used for the
-native-accumulation path (fp32 partial sums per slice, one fp32 add per
-slice).
+One SpMV over the compact rows (entry tiles) writes a partial z[i] per compact
+row.  Its tiles are dealt to 8 block groups -- group g takes the tiles of
+slices [g*q, (g+1)*q) in order -- and blocks are dispatched to the XCDs
+round-robin, so XCD g gathers only from its own x columns, one slice at a
+time.  ``fs_sliced_combine`` then adds a row's partials in slice order (fp64)
+and applies alpha / beta.  The result is deterministic.
+
+Prepare cost: one stable sort of the entries by (slice, row) on the device,
+and a second copy of col / values in slice-major order (the prepared object
+keeps both layouts).
 """
 
 from __future__ import annotations
@@ -23,53 +26,87 @@ import math
 import torch
 
 from . import _lib
-from .rowsplit import RowSplitPlan
+from .rowsplit import RowSplitPlan, Schedule
+
+N_GROUPS = 8          # MI355X: 8 XCDs, workgroups dealt round-robin
+COMBINE_ROWS = 256    # rows per block of the combine pass
+SLICED_DTYPES = (torch.float32, torch.float64, torch.complex64, torch.complex128)
+
+
+def slice_count(n_cols: int, slice_cols: int) -> int:
+    """P = 8q slices of at most slice_cols columns (q >= 1)."""
+    q = max(1, math.ceil(n_cols / (N_GROUPS * max(int(slice_cols), 1))))
+    return N_GROUPS * q
 
 
 class SlicedCsr:
-    """Per-slice CSR matrices and plans of one matrix (see module doc)."""
+    """Slice-major copy of one CSR matrix plus its compact-row plan."""
 
     def __init__(self, data, indices, indptr, n_rows, n_cols, slice_cols, schedule,
                  accumulate):
+        if data.dtype not in SLICED_DTYPES:
+            raise TypeError(f"column slicing supports {SLICED_DTYPES}, got {data.dtype}")
         dev = data.device
         self.n_rows, self.n_cols = int(n_rows), int(n_cols)
-        self.n_slices = max(1, math.ceil(self.n_cols / max(int(slice_cols), 1)))
-        self.width = math.ceil(self.n_cols / self.n_slices) if self.n_cols else 0
-        w, P, n = self.width, self.n_slices, self.n_rows
+        n = self.n_rows
+        P = slice_count(self.n_cols, slice_cols)
+        self.n_slices = P
+        self.width = max(1, math.ceil(self.n_cols / P))
+        w = self.width
         nnz = data.numel()
         lens = (indptr[1:] - indptr[:-1]).to(torch.int64)
         rows = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int64), lens)
-        cols = indices.to(torch.int64)
-        sl = cols // w if w else torch.zeros_like(cols)
+        sl = indices.to(torch.int64) // w
         key = sl * n + rows
-        order = torch.argsort(key, stable=True)
         del rows
-        sl_sorted = sl[order]
-        self.cols = (cols[order] - sl_sorted * w).to(torch.int32)
+        order = torch.argsort(key, stable=True)
+        key = key[order]
+        self.cols = indices[order].contiguous()
         self.vals = data[order].contiguous()
-        del cols, sl, order, sl_sorted
-        counts = torch.bincount(key, minlength=P * n).view(P, n)
+        del order
+        ukeys, counts = torch.unique_consecutive(key, return_counts=True)
         del key
-        ptr_dtype = torch.int32 if nnz <= 2**31 - 1 else torch.int64
-        self.parts = []
-        start = 0
-        for k in range(P):
-            ip = torch.zeros(n + 1, dtype=torch.int64, device=dev)
-            torch.cumsum(counts[k], 0, out=ip[1:])
-            nk = int(ip[-1].item())
-            ip = ip.to(ptr_dtype)
-            c0 = k * w
-            wk = max(0, min(w, self.n_cols - c0))
-            cols_k = self.cols[start:start + nk]
-            vals_k = self.vals[start:start + nk]
-            csr = _lib.make_csr(vals_k, cols_k, ip, n, wk)
-            plan = RowSplitPlan(ip, n, data.dtype, schedule, accumulate,
-                                row_lengths=counts[k].to(torch.int64))
-            self.parts.append((csr, plan, ip, c0, wk))
-            start += nk
+        self.n_compact = int(ukeys.numel())
+        ptr64 = torch.zeros(self.n_compact + 1, dtype=torch.int64, device=dev)
+        torch.cumsum(counts, 0, out=ptr64[1:])
+        self.cptr = ptr64.to(torch.int32) if nnz <= 2**31 - 1 else ptr64
+        self.compact_row = (ukeys % max(n, 1)).to(torch.int32)
+        # first entry of each slice (entries are slice-major)
+        slice_of_compact = ukeys // max(n, 1)
+        per_slice = torch.zeros(P, dtype=torch.int64, device=dev)
+        per_slice.index_add_(0, slice_of_compact, counts)
+        slice_entry = torch.zeros(P + 1, dtype=torch.int64, device=dev)
+        torch.cumsum(per_slice, 0, out=slice_entry[1:])
+        # combine bounds: first compact row of slice k in row block b
+        n_blocks = -(-n // COMBINE_ROWS)
+        q = (torch.arange(P, device=dev, dtype=torch.int64)[:, None] * n
+             + torch.arange(n_blocks + 1, device=dev, dtype=torch.int64)[None, :] * COMBINE_ROWS)
+        q = torch.minimum(q, (torch.arange(P, device=dev, dtype=torch.int64)[:, None] + 1) * n)
+        self.bounds = torch.searchsorted(ukeys, q.reshape(-1).contiguous(), side="left")
+        del ukeys, slice_of_compact
+        # entry tiles over the compact rows, dealt to XCD groups by slice
+        T = schedule.tile_nnz if (schedule.tile_nnz > 0 and schedule.tile_mode == 1) else (
+            4096 if data.dtype == torch.float32 else 2048)
+        self.schedule = Schedule(tile_nnz=T, tile_mode=1)
+        self.plan = RowSplitPlan(self.cptr, self.n_compact, data.dtype, self.schedule, accumulate)
+        qn = P // N_GROUPS
+        first = slice_entry[torch.arange(0, P, qn, device=dev)] // T
+        gt = torch.empty(N_GROUPS + 1, dtype=torch.int64, device=dev)
+        gt[:N_GROUPS] = torch.clamp(first, max=self.plan.n_tiles)
+        gt[N_GROUPS] = self.plan.n_tiles
+        gt[0] = 0
+        self.group_tiles = gt
+        self.group_blocks = int((gt[1:] - gt[:-1]).max().item())
+        self.plan.set_groups(N_GROUPS, self.group_blocks, self.group_tiles)
+        self.csr = _lib.make_csr(self.vals, self.cols, self.cptr, self.n_compact, self.n_cols)
+        self.z = torch.empty(max(self.n_compact, 1), dtype=data.dtype, device=dev)
+        self.desc = _lib.FsSlices(value_type=_lib.value_code(data.dtype), n_slices=P,
+                                  n_rows=n, n_compact=self.n_compact,
+                                  compact_row=_lib.ptr(self.compact_row),
+                                  bounds=_lib.ptr(self.bounds), rows_per_block=COMBINE_ROWS)
 
     def run(self, x, y, alpha, beta, device):
-        """y = alpha A x + beta y, one launch (plus fix-up) per slice."""
-        for k, (csr, plan, _ip, c0, wk) in enumerate(self.parts):
-            xk = x[c0:c0 + wk]
-            _lib.spmv(csr, xk, y, alpha, beta if k == 0 else 1.0, plan.struct(1), device)
+        """y = alpha A x + beta y: compact-row partials, then the combine."""
+        if self.n_compact > 0:
+            _lib.spmv(self.csr, x, self.z, 1.0, 0.0, self.plan.struct(1), device)
+        _lib.sliced_combine(self.desc, self.z, y, alpha, beta, device)

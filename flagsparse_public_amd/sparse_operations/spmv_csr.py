@@ -25,7 +25,7 @@ import torch
 from .. import _lib
 from ..rowsplit import RowSplitPlan, Schedule, default_spmv_schedule
 from ..schedule_choice import choose_schedule, exhaustive_spmv_schedule
-from ..slicing import SlicedCsr
+from ..slicing import SLICED_DTYPES, SlicedCsr
 from ._common import (
     SUPPORTED_INDEX_DTYPES,
     _INDEX_LIMIT_INT32,
@@ -133,7 +133,7 @@ class PreparedCsrSpmv:
 
     def plan(self, accumulate, schedule=None):
         sch = schedule or self.schedule or default_spmv_schedule(self.n_rows, self.data.numel(),
-                                                                 self.data.dtype)
+                                                                 self.data.dtype, self.n_cols)
         key = (int(accumulate), sch.key())
         p = self._plans.get(key)
         if p is None:
@@ -360,7 +360,7 @@ def _launch_spmv(prepared, x, accumulate, out=None, alpha=1.0, beta=0.0, schedul
         return y
     sch = schedule or prepared.schedule
     if (sch is not None and sch.x_slice_cols > 0 and prepared.n_cols > sch.x_slice_cols
-            and accumulate == _lib.FS_ACC_NATIVE):
+            and accumulate == _lib.FS_ACC_NATIVE and prepared.data.dtype in SLICED_DTYPES):
         key = ("sliced", sch.key())
         sliced = prepared._plans.get(key)
         if sliced is None:

@@ -28,7 +28,7 @@ import torch
 
 from . import _lib, synthetic
 from .autotune import (SPMM_DEFAULT_INDEX, SPMM_VARIANTS, SPMV_DEFAULT_INDEX, SPMV_VARIANTS,
-                       TimingDB, tune_spmm, tune_spmv)
+                       TimingDB, spmv_heuristic_index, tune_spmm, tune_spmv)
 from .selector import Selector
 from .sparse_operations.spmm_csr import prepare_spmm_csr_opt
 from .sparse_operations.spmv_csr import prepare_spmv_csr
@@ -41,7 +41,7 @@ def _log(msg):
     print(f"[train_selector] {msg}", flush=True)
 
 
-def collect(seeds, device, dbs, max_n, reps, t0, label):
+def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=100):
     dense_cache = {}
     for k, seed in enumerate(seeds):
         family, n = synthetic.selector_matrix_spec(seed)
@@ -63,14 +63,19 @@ def collect(seeds, device, dbs, max_n, reps, t0, label):
         tm = tune_spmm(pm, dense_cache[n], reps=max(3, reps - 2))
         dbs["spmm64"].add(seed, family, feats, tm, n, nnz)
         del pv, pm, indptr, cols, vals, x
+        if save is not None and (k + 1) % save_every == 0:
+            save()
         if (k + 1) % 20 == 0 or k + 1 == len(seeds):
             _log(f"{label} {k + 1}/{len(seeds)} matrices, {time.time() - t0:.0f}s "
                  f"(last: {family} n={n} nnz={nnz}, spmv best {tv.min():.1f}us, "
                  f"spmm best {tm.min():.1f}us)")
 
 
-def evaluate(sel: Selector, db: TimingDB, default_index: int, device):
-    _, fams, feats, times, _ = db.arrays()
+def evaluate(sel: Selector, db: TimingDB, default_index: int, device, heuristic=None):
+    """Regret against exhaustive autotuning and speed-up over the plain
+    default-tile variant (and over the fixed heuristic rule when given: a
+    function n_cols -> variant index)."""
+    _, fams, feats, times, sizes = db.arrays()
     choices = []
     for f in feats:
         fd = torch.as_tensor(f.astype(np.float32), device=device)
@@ -92,6 +97,11 @@ def evaluate(sel: Selector, db: TimingDB, default_index: int, device):
         "exhaustive_geomean_speedup_vs_default": float(np.exp(np.mean(np.log(t_def / t_best)))),
         "per_family": {},
     }
+    if heuristic is not None:
+        hidx = np.asarray([heuristic(int(n)) for n in sizes[:, 0]])
+        t_heu = times[rows, hidx]
+        out["geomean_speedup_vs_heuristic"] = float(np.exp(np.mean(np.log(t_heu / t_sel))))
+        out["heuristic_geomean_regret"] = float(np.exp(np.mean(np.log(t_heu / t_best))))
     for fam in sorted(set(fams.tolist())):
         m = fams == fam
         out["per_family"][fam] = {
@@ -128,10 +138,12 @@ def main(argv=None):
                     "spmm64": TimingDB("spmm64", len(SPMM_VARIANTS), name)}
         done = set(have["spmv"].ids)
         seeds = [s for s in range(seed0, seed0 + count) if s not in done]
-        if seeds:
-            collect(seeds, device, have, args.max_n, args.reps, t0, split)
+        def save(have=have, split=split):
             for op, db in have.items():
                 db.save(paths[split][op])
+        if seeds:
+            collect(seeds, device, have, args.max_n, args.reps, t0, split, save=save)
+            save()
         dbs[split] = have
     if args.collect_only:
         return 0
@@ -142,8 +154,9 @@ def main(argv=None):
         sel = Selector.train(op, Xtr, Ttr, device, epochs=args.epochs,
                              log=lambda m, op=op: _log(f"{op}: {m}"))
         sel.meta.update({"device": name, "default_index": default_index})
-        ev, _ = evaluate(sel, dbs["test"][op], default_index, device)
-        tr, _ = evaluate(sel, dbs["train"][op], default_index, device)
+        heur = spmv_heuristic_index if op == "spmv" else None
+        ev, _ = evaluate(sel, dbs["test"][op], default_index, device, heur)
+        tr, _ = evaluate(sel, dbs["train"][op], default_index, device, heur)
         report[op] = {"held_out": ev, "train": {k: tr[k] for k in
                                                 ("geomean_regret", "geomean_speedup_vs_default")}}
         sel.meta["held_out"] = ev

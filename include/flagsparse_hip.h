@@ -118,7 +118,64 @@ typedef struct fs_plan {
   int64_t tile_nnz;              /* > 0: tile role for short rows (SpMV)   */
   int64_t n_tiles;
   const int64_t *tile_start;     /* [n_tiles + 1] first row of each tile   */
+  int32_t tile_mode;             /* FS_TILE_ROWS or FS_TILE_ENTRIES        */
+  int32_t n_groups;              /* FS_TILE_ENTRIES: 0, or block groups    */
+  int64_t group_blocks;          /* blocks per group (largest tile range)  */
+  const int64_t *group_tiles;    /* [n_groups + 1] tile range of a group   */
 } fs_plan;
+
+/*
+ * Tile modes of the SpMV (tile_nnz > 0):
+ *   FS_TILE_ROWS     tiles of consecutive short rows holding about tile_nnz
+ *                    entries; rows over short_max go to the listed long-row
+ *                    chunks.
+ *   FS_TILE_ENTRIES  tile t holds exactly the entries [t*tile_nnz,
+ *                    (t+1)*tile_nnz) and owns the rows that start in it;
+ *                    n_tiles = max(1, ceil(nnz / tile_nnz)), tile_start[t] =
+ *                    first owned row (tile_start[n_tiles] = n_rows), no long-
+ *                    row lists (n_long = 0), partials = two carries per tile
+ *                    (fs_tile_carry_bytes).  tile_nnz is 1024, 2048 or 4096
+ *                    for fp32 / fp64 values with int32 columns, 2048 otherwise.
+ *                    With n_groups = G > 0, the grid is G * group_blocks
+ *                    blocks; block b serves group g = b % G and takes tile
+ *                    group_tiles[g] + b / G (blocks past group_tiles[g + 1]
+ *                    exit; the ranges must cover every tile once and
+ *                    group_blocks must be the largest range): blocks are dispatched to the
+ *                    8 XCDs round-robin, so G = 8 keeps each XCD on its own
+ *                    contiguous tile range -- and its L2 on its own x columns
+ *                    when the matrix is stored column-slice-major.
+ */
+enum { FS_TILE_ROWS = 0, FS_TILE_ENTRIES = 1 };
+
+/* Bytes of `partials` for FS_TILE_ENTRIES with n_tiles tiles. */
+int64_t fs_tile_carry_bytes(int32_t value_type, int64_t n_tiles);
+
+/* ------------------------------------------------- column-sliced SpMV */
+/*
+ * No counterpart in the reference (whose SpMV gathers x in place,
+ * spmv_csr.py:158-184); used by the fp32 / fp64 / complex native path when x
+ * is larger than one XCD's L2.  The matrix is stored column-slice-major:
+ * every (slice, row) pair with entries is a compact row, ordered by slice,
+ * then row.  fs_spmv_csr over the compact rows (absolute column indices)
+ * writes one partial per compact row into z; fs_sliced_combine then forms
+ * y[r] = alpha * sum over slices of z[compact row (k, r)] + beta * y[r],
+ * summed in slice order in fp64.
+ */
+typedef struct fs_slices {
+  int32_t value_type;        /* FS_F32, FS_F64, FS_C64 or FS_C128          */
+  int32_t n_slices;
+  int64_t n_rows;            /* rows of the original matrix                */
+  int64_t n_compact;
+  const int32_t *compact_row;  /* [n_compact] row of each compact row      */
+  const int64_t *bounds;     /* [n_slices * (n_blocks + 1)], n_blocks =
+                                ceil(n_rows / rows_per_block): first compact
+                                row of slice k in row block b               */
+  int32_t rows_per_block;    /* 256                                        */
+  int32_t pad;
+} fs_slices;
+
+int fs_sliced_combine(const fs_slices *sl, const void *z, void *y, double alpha, double beta,
+                      void *stream);
 
 /* Bytes of `partials` needed for n_chunks chunks (n_dense = 1 for SpMV). */
 int64_t fs_partials_bytes(int32_t value_type, int32_t accumulate, int64_t n_chunks,

@@ -125,13 +125,16 @@ SCHEDULES = [
         (2048, 256, 8192, 8, 1), (512, 0, 64, 4, 0),
     ]
 ] + [
+    # entry tiles: fixed runs of entries, rows cut at tile boundaries
+    Schedule(tile_nnz=t, tile_mode=1) for t in (1024, 2048, 4096)
+] + [
     # column slices (native path): 7 and 3 slices of the 20000 columns
     Schedule(tile_nnz=1024, short_max=1024, chunk_nnz=2048, unroll=4, x_slice_cols=3000),
     Schedule(group_lanes=8, short_max=128, chunk_nnz=512, unroll=4, x_slice_cols=7000),
 ]
 
 
-@pytest.mark.parametrize("sch", SCHEDULES, ids=lambda s: f"S{s.group_lanes}_t{s.tile_nnz}_m{s.short_max}_c{s.chunk_nnz}_u{s.unroll}_x{s.xcd_remap}_sl{s.x_slice_cols}")
+@pytest.mark.parametrize("sch", SCHEDULES, ids=lambda s: f"S{s.group_lanes}_t{s.tile_nnz}_m{s.short_max}_c{s.chunk_nnz}_u{s.unroll}_x{s.xcd_remap}_sl{s.x_slice_cols}_tm{s.tile_mode}")
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
 @pytest.mark.parametrize("ptr_dtype", [torch.int32, torch.int64], ids=["p32", "p64"])
 def test_spmv_schedules_powerlaw(orc, sch, dtype, ptr_dtype):
@@ -152,6 +155,65 @@ def test_spmv_schedules_powerlaw(orc, sch, dtype, ptr_dtype):
     for _ in range(3):
         assert torch.equal(flagsparse_spmv_csr(x=xd, prepared=p, use_opt=True), y_opt)
         assert torch.equal(flagsparse_spmv_csr(x=xd, prepared=p), y_def)
+
+
+@pytest.mark.parametrize("dtype", [torch.float16, torch.bfloat16, torch.float32, torch.float64,
+                                   torch.complex64, torch.complex128],
+                         ids=["f16", "bf16", "f32", "f64", "c64", "c128"])
+@pytest.mark.parametrize("index_dtype", [torch.int32, torch.int64], ids=["i32", "i64"])
+def test_spmv_entry_tiles_all_dtypes(orc, dtype, index_dtype):
+    """Entry tiles (2048 entries; 1024 and 4096 where built) on a power-law
+    matrix with rows far longer than a tile, for every value / index dtype,
+    both accumulation paths, alpha / beta, and misaligned arrays (scalar
+    loads instead of 16-byte ones)."""
+    npd = {torch.float16: np.float32, torch.bfloat16: np.float32, torch.float32: np.float32,
+           torch.float64: np.float64, torch.complex64: np.complex64,
+           torch.complex128: np.complex128}[dtype]
+    ip, cols, vals = _powerlaw(9000, 23, 20000, np.float64)
+    n = ip.size - 1
+    rng = np.random.default_rng(31)
+    x = rng.uniform(-1, 1, n)
+    if dtype.is_complex:
+        vals = vals + 1j * rng.uniform(-1, 1, vals.size)
+        x = x + 1j * rng.uniform(-1, 1, n)
+    vals = vals.astype(npd)
+    x = x.astype(npd)
+    data = to_dev(vals).to(dtype)
+    xd = to_dev(x).to(dtype)
+    # oracle on the values as stored (fp16 / bf16 rounding included)
+    vs = data.cpu().to(torch.complex128 if dtype.is_complex else torch.float64).numpy()
+    xs = xd.cpu().to(torch.complex128 if dtype.is_complex else torch.float64).numpy()
+    y64 = orc.spmv(ip, cols.astype(np.int64), vs, xs)
+    cond = orc.spmv_abs(ip, cols.astype(np.int64), vs, xs)
+    hot = dtype in (torch.float32, torch.float64) and index_dtype == torch.int32
+    sizes = (1024, 2048, 4096) if hot else (2048,)
+    for T in sizes:
+        for misalign in (False, True):
+            if misalign:  # one element of padding in front: no 16-byte loads
+                d2 = torch.empty(data.numel() + 1, dtype=dtype, device=DEV)[1:]
+                d2.copy_(data)
+                c2 = torch.empty(cols.size + 1, dtype=index_dtype, device=DEV)[1:]
+                c2.copy_(to_dev(cols).to(index_dtype))
+            else:
+                d2, c2 = data, to_dev(cols).to(index_dtype)
+            sch = Schedule(tile_nnz=T, tile_mode=1)
+            p = prepare_spmv_csr(d2, c2, to_dev(ip), (n, n), schedule=sch)
+            for use_opt in ((False, True) if dtype in (torch.float32, torch.float64) else (False,)):
+                y = flagsparse_spmv_csr(x=xd, prepared=p, use_opt=use_opt)
+                got = y.cpu().to(torch.complex128 if dtype.is_complex else torch.float64).numpy()
+                err = row_rel_err(got, y64, cond).max()
+                bound = {torch.float16: 1e-3, torch.bfloat16: 8e-3, torch.float32: 1e-6,
+                         torch.float64: 1e-13, torch.complex64: 1e-6, torch.complex128: 1e-13}[dtype]
+                assert err <= bound, (T, misalign, use_opt, err)
+            # y = 2 A x - 0.5 y_old
+            y0 = to_dev(rng.uniform(-1, 1, n).astype(npd)).to(dtype)
+            y1 = y0.clone()
+            spmv_mod._hip_spmv_csr_impl_prepared(p, xd, out=y1, alpha=2.0, beta=-0.5)
+            want = 2.0 * y64 - 0.5 * y0.cpu().to(torch.complex128 if dtype.is_complex else torch.float64).numpy()  # noqa: E501
+            got = y1.cpu().to(torch.complex128 if dtype.is_complex else torch.float64).numpy()
+            y0h = y0.cpu().to(torch.complex128 if dtype.is_complex else torch.float64).numpy()
+            scale_ = 2.0 * cond + 0.5 * np.abs(y0h)
+            assert (np.abs(got - want) / np.maximum(scale_, 1e-300)).max() <= bound
 
 
 def test_spmv_long_rows_compensated_fp32(orc):
@@ -192,8 +254,10 @@ def test_spmv_tiles_many_empty_rows(orc, dtype):
         x = (x + 1j * rng.uniform(-1, 1, m)).astype(npd)
     y64 = orc.spmv(ip, cols, vals, x)
     cond = orc.spmv_abs(ip, cols, vals, x)
-    for sch in (Schedule(tile_nnz=2048, short_max=2048, chunk_nnz=4096, unroll=8),
-                Schedule(tile_nnz=256, short_max=256, chunk_nnz=256, unroll=4)):
+    big = 1024 if dtype.is_complex else 2048  # complex row tiles: 16-byte sums in LDS
+    for sch in (Schedule(tile_nnz=big, short_max=big, chunk_nnz=4096, unroll=8),
+                Schedule(tile_nnz=256, short_max=256, chunk_nnz=256, unroll=4),
+                Schedule(tile_nnz=2048, tile_mode=1)):
         p = prepare_spmv_csr(to_dev(vals), to_dev(cols), to_dev(ip), (n, m), schedule=sch)
         for use_opt in (False, True):
             y = flagsparse_spmv_csr(x=to_dev(x), prepared=p, use_opt=use_opt)

@@ -219,7 +219,14 @@ def test_default_schedules():
     assert rowsplit.rowgroup_spmv_schedule(10, 10).group_lanes == 1
     assert rowsplit.rowgroup_spmv_schedule(10, 100000).group_lanes == 64
     d = rowsplit.default_spmv_schedule(4_000_000, 48_000_000)
-    assert d.tile_nnz > 0 and d.short_max <= d.tile_nnz and d.chunk_nnz % 64 == 0
+    assert d.tile_nnz == 2048 and d.tile_mode == 1 and d.x_slice_cols == 0
+    # x of 16 MB (fp32): slices of 2 MB of x; 2 MB x: no slicing
+    d = rowsplit.default_spmv_schedule(4_000_000, 48_000_000, torch.float32, 4_000_000)
+    assert d.x_slice_cols == 512 * 1024
+    assert rowsplit.default_spmv_schedule(500_000, 10, torch.float32, 500_000).x_slice_cols == 0
+    assert rowsplit.default_spmv_schedule(10, 10, torch.float64, 600_000).x_slice_cols == 256 * 1024
+    r = rowsplit.row_tile_spmv_schedule(10, 10, torch.complex64)
+    assert r.tile_nnz == 1024 and r.short_max <= r.tile_nnz and r.chunk_nnz % 64 == 0
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
@@ -239,6 +246,37 @@ def test_tile_starts_invariants(seed):
                 assert b - a == 1  # a long row is a tile of its own
             else:
                 assert seg.sum() < tile + short_max
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("T", [1024, 2048, 4096])
+def test_entry_tile_ownership(seed, T):
+    """Entry tiles: tile t owns exactly the rows whose first entry lies in
+    [t*T, (t+1)*T); the last tile also owns the rows that start at nnz; every
+    row is owned once (brute force over the rows)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.poisson(5, 6000)
+    lens[rng.random(6000) < 0.3] = 0
+    lens[rng.integers(0, 6000, 4)] = [9000, 4096, 2 * T, 1]
+    lens[-50:] = 0  # trailing empty rows start at nnz
+    ip = np.zeros(lens.size + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    nnz = int(ip[-1])
+    sch = rowsplit.Schedule(tile_nnz=T, tile_mode=1)
+    plan = rowsplit.RowSplitPlan(torch.from_numpy(ip), lens.size, torch.float32, sch, 0)
+    ts = plan.tile_start.tolist()
+    assert plan.n_tiles == max(1, -(-nnz // T)) and len(ts) == plan.n_tiles + 1
+    assert ts[0] == 0 and ts[-1] == lens.size and ts == sorted(ts)
+    owner = np.minimum(ip[:-1] // T, plan.n_tiles - 1)
+    for t in range(plan.n_tiles):
+        assert np.all(owner[ts[t]:ts[t + 1]] == t)
+    assert plan.n_long == 0
+
+
+def test_entry_tile_empty_matrix():
+    ip = torch.zeros(7, dtype=torch.int64)
+    plan = rowsplit.RowSplitPlan(ip, 6, torch.float64, rowsplit.Schedule(tile_nnz=2048, tile_mode=1), 0)
+    assert plan.n_tiles == 1 and plan.tile_start.tolist() == [0, 6]
 
 
 def test_synthetic_generators_deterministic_and_sorted():
