@@ -279,6 +279,43 @@ def test_entry_tile_empty_matrix():
     assert plan.n_tiles == 1 and plan.tile_start.tolist() == [0, 6]
 
 
+def test_sliced_layout_structure():
+    """The slice-major copy: compact rows ordered by (slice, row), every entry
+    in its slice, the compact-row sum over rows equal to A x (fp64, SciPy),
+    combine bounds selecting (slice, 256-row block) ranges, and the XCD group
+    ranges covering every tile once."""
+    import scipy.sparse as sp
+    from flagsparse_public_amd import slicing
+    ip, cols, vals = synthetic.powerlaw_csr(20000, 11, alpha=1.6, max_len=30000)
+    n = ip.size - 1
+    s = slicing.SlicedCsr(torch.from_numpy(vals), torch.from_numpy(cols), torch.from_numpy(ip),
+                          n, n, 3000, rowsplit.Schedule(), 0)
+    assert s.n_slices == 8 and s.width == 2500
+    x = np.random.default_rng(0).uniform(-1, 1, n)
+    cp = s.cptr.numpy().astype(np.int64)
+    cr = s.compact_row.numpy()
+    sc = s.cols.numpy().astype(np.int64)
+    sv = s.vals.numpy().astype(np.float64)
+    z = np.add.reduceat(sv * x[sc], cp[:-1]) if s.n_compact else np.zeros(0)
+    y = np.zeros(n)
+    np.add.at(y, cr, z)
+    ref = sp.csr_matrix((vals.astype(np.float64), cols, ip), shape=(n, n)) @ x
+    assert np.abs(y - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    sl = sc[cp[:-1]] // s.width
+    key = sl * n + cr
+    assert np.all(np.diff(key) > 0)
+    assert np.all(np.repeat(sl, np.diff(cp)) == sc // s.width)
+    b = s.bounds.numpy().reshape(s.n_slices, -1)
+    for k in range(s.n_slices):
+        for bb in range(b.shape[1] - 1):
+            seg = np.arange(b[k, bb], b[k, bb + 1])
+            assert seg.size <= slicing.COMBINE_ROWS
+            assert np.all(sl[seg] == k) and np.all(cr[seg] // slicing.COMBINE_ROWS == bb)
+    gt = s.group_tiles.tolist()
+    assert gt[0] == 0 and gt[-1] == s.plan.n_tiles and gt == sorted(gt)
+    assert s.group_blocks == max(np.diff(gt))
+
+
 def test_synthetic_generators_deterministic_and_sorted():
     a = synthetic.poisson_csr(2000, 12.0, 9)
     b = synthetic.poisson_csr(2000, 12.0, 9)
