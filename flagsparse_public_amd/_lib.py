@@ -248,6 +248,26 @@ def make_csr(data, indices, indptr, n_rows, n_cols) -> FsCsr:
     )
 
 
+class _Nothing:
+    def __enter__(self):
+        return None
+
+    def __exit__(self, *exc):
+        return False
+
+
+_NOTHING = _Nothing()
+
+
+def on_device(device):
+    """Context making `device` current; a no-op when it already is (the common
+    case: one GPU per process), which keeps the per-call host cost low."""
+    idx = device.index if isinstance(device, torch.device) else int(device)
+    if idx is None or idx == torch.cuda.current_device():
+        return _NOTHING
+    return torch.cuda.device(idx)
+
+
 def stream_handle(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
@@ -256,7 +276,7 @@ def analyze_device(csr: FsCsr, device):
     """Run fs_csr_analyze; returns the FS_STATS_LEN statistics on the device."""
     lib = load()
     stats = torch.empty(FS_STATS_LEN, dtype=torch.int64, device=device)
-    with torch.cuda.device(device):
+    with on_device(device):
         check(lib.fs_csr_analyze(ctypes.byref(csr), stats.data_ptr(), stream_handle(device)),
               "fs_csr_analyze")
     return stats
@@ -270,7 +290,7 @@ def analyze(csr: FsCsr, device) -> list[int]:
 def features(stats, n_rows: int, n_cols: int, nnz: int, device):
     """16 selector features (device float32) from device statistics."""
     f = torch.empty(16, dtype=torch.float32, device=device)
-    with torch.cuda.device(device):
+    with on_device(device):
         check(load().fs_csr_features(stats.data_ptr(), int(n_rows), int(n_cols), int(nnz),
                                      f.data_ptr(), stream_handle(device)), "fs_csr_features")
     return f
@@ -282,13 +302,13 @@ def mlp_struct(params, in_dim: int, hidden: int, out_dim: int) -> FsMlp:
 
 
 def mlp_forward(net: FsMlp, X, H1, H2, Y, device) -> None:
-    with torch.cuda.device(device):
+    with on_device(device):
         check(load().fs_mlp_forward(ctypes.byref(net), ptr(X), int(X.shape[0]), ptr(H1), ptr(H2),
                                     ptr(Y), stream_handle(device)), "fs_mlp_forward")
 
 
 def mlp_backward(net: FsMlp, X, H1, H2, Y, T, M, grad, loss, workspace, device) -> None:
-    with torch.cuda.device(device):
+    with on_device(device):
         check(load().fs_mlp_backward(ctypes.byref(net), ptr(X), ptr(H1), ptr(H2), ptr(Y), ptr(T),
                                      ptr(M), int(X.shape[0]), ptr(grad), ptr(loss), ptr(workspace),
                                      int(workspace.numel()), stream_handle(device)),
@@ -304,21 +324,21 @@ def mlp_param_count(in_dim, hidden, out_dim) -> int:
 
 
 def adam_step(params, grad, m, v, lr, beta1, beta2, eps, step, device) -> None:
-    with torch.cuda.device(device):
+    with on_device(device):
         check(load().fs_adam_step(ptr(params), ptr(grad), ptr(m), ptr(v), int(params.numel()),
                                   float(lr), float(beta1), float(beta2), float(eps), int(step),
                                   stream_handle(device)), "fs_adam_step")
 
 
 def mlp_select(net: FsMlp, feats, mu, sd, choice, scores, device) -> None:
-    with torch.cuda.device(device):
+    with on_device(device):
         check(load().fs_mlp_select(ctypes.byref(net), ptr(feats), ptr(mu), ptr(sd), ptr(choice),
                                    ptr(scores), stream_handle(device)), "fs_mlp_select")
 
 
 def spmv(csr: FsCsr, x, y, alpha: float, beta: float, plan: FsPlan, device) -> None:
     lib = load()
-    with torch.cuda.device(device):
+    with on_device(device):
         check(lib.fs_spmv_csr(ctypes.byref(csr), ptr(x), ptr(y), float(alpha), float(beta),
                               ctypes.byref(plan), stream_handle(device)), "fs_spmv_csr")
 
@@ -326,7 +346,7 @@ def spmv(csr: FsCsr, x, y, alpha: float, beta: float, plan: FsPlan, device) -> N
 def spmm(csr: FsCsr, B, ldb: int, C, ldc: int, n_dense: int, alpha: float, beta: float,
          plan: FsPlan, col_tile: int, device) -> None:
     lib = load()
-    with torch.cuda.device(device):
+    with on_device(device):
         check(lib.fs_spmm_csr(ctypes.byref(csr), ptr(B), int(ldb), ptr(C), int(ldc),
                               int(n_dense), float(alpha), float(beta), ctypes.byref(plan),
                               int(col_tile), stream_handle(device)), "fs_spmm_csr")
@@ -334,7 +354,7 @@ def spmm(csr: FsCsr, B, ldb: int, C, ldc: int, n_dense: int, alpha: float, beta:
 
 def split_values(fmt: int, csr: FsCsr, hi, lo, row_scale, device) -> None:
     lib = load()
-    with torch.cuda.device(device):
+    with on_device(device):
         check(lib.fs_split_values(int(fmt), ctypes.byref(csr), ptr(hi), ptr(lo), ptr(row_scale),
                                   stream_handle(device)), "fs_split_values")
 
@@ -342,7 +362,7 @@ def split_values(fmt: int, csr: FsCsr, hi, lo, row_scale, device) -> None:
 def spmv_split(csr: FsCsr, fmt: int, hi, lo, row_scale, x, y, alpha: float, beta: float,
                plan: FsPlan, device) -> None:
     lib = load()
-    with torch.cuda.device(device):
+    with on_device(device):
         check(lib.fs_spmv_csr_split(ctypes.byref(csr), int(fmt), ptr(hi), ptr(lo), ptr(row_scale),
                                     ptr(x), ptr(y), float(alpha), float(beta), ctypes.byref(plan),
                                     stream_handle(device)), "fs_spmv_csr_split")

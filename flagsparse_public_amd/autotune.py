@@ -74,16 +74,43 @@ SPMM_VARIANTS = (
 SPMM_DEFAULT_INDEX = 0
 
 
+HOLD_US = 1500.0  # device time the queue is held while the host enqueues a group
+_hold_cycles = None
+
+
+def _hold():
+    """Cycles of torch.cuda._sleep that keep the stream busy for about
+    HOLD_US, measured once per process."""
+    global _hold_cycles
+    if _hold_cycles is None:
+        probe = 2_000_000
+        torch.cuda._sleep(probe)  # warm the spin kernel
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        stream = torch.cuda.current_stream()
+        a.record(stream)
+        torch.cuda._sleep(probe)
+        b.record(stream)
+        b.synchronize()
+        us = max(a.elapsed_time(b) * 1000.0, 1.0)
+        _hold_cycles = max(1, int(probe * HOLD_US / us))
+    return _hold_cycles
+
+
 def time_launches(fn, reps: int = 7, warmup: int = 2, inner: int = 1) -> float:
-    """Median microseconds per call of fn() over reps event-timed groups of
-    ``inner`` calls on the current stream."""
+    """Median device microseconds per call of fn() over reps event-timed
+    groups of ``inner`` calls on the current stream.  Each group is enqueued
+    behind a short spin kernel, so the launches run back to back and the
+    timing is the device's, not the host's launch rate."""
     for _ in range(warmup):
         fn()
     stream = torch.cuda.current_stream()
+    hold = _hold()
     samples = []
     for _ in range(reps):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(hold)
         a.record(stream)
         for _ in range(inner):
             fn()

@@ -35,8 +35,9 @@ class Schedule:
                               # at most this many (one launch per slice, x slice in L2)
     tile_mode: int = 0        # SpMV with tile_nnz > 0: 0 row tiles, 1 entry tiles
 
-    def key(self) -> tuple:
-        return dataclasses.astuple(self)
+    def key(self) -> "Schedule":
+        """Cache key (a frozen dataclass hashes and compares by its fields)."""
+        return self
 
 
 def _pow2_ceil(v: float) -> int:

@@ -71,19 +71,18 @@ class SlicedCsr:
         torch.cumsum(counts, 0, out=ptr64[1:])
         self.cptr = ptr64.to(torch.int32) if nnz <= 2**31 - 1 else ptr64
         self.compact_row = (ukeys % max(n, 1)).to(torch.int32)
-        # first entry of each slice (entries are slice-major)
-        slice_of_compact = ukeys // max(n, 1)
-        per_slice = torch.zeros(P, dtype=torch.int64, device=dev)
-        per_slice.index_add_(0, slice_of_compact, counts)
-        slice_entry = torch.zeros(P + 1, dtype=torch.int64, device=dev)
-        torch.cumsum(per_slice, 0, out=slice_entry[1:])
+        # first entry of each slice (entries are slice-major): the entry offset
+        # of the slice's first compact row
+        first_compact = torch.searchsorted(
+            ukeys, torch.arange(P + 1, device=dev, dtype=torch.int64) * max(n, 1), side="left")
+        slice_entry = ptr64[first_compact]
         # combine bounds: first compact row of slice k in row block b
         n_blocks = -(-n // COMBINE_ROWS)
         q = (torch.arange(P, device=dev, dtype=torch.int64)[:, None] * n
              + torch.arange(n_blocks + 1, device=dev, dtype=torch.int64)[None, :] * COMBINE_ROWS)
         q = torch.minimum(q, (torch.arange(P, device=dev, dtype=torch.int64)[:, None] + 1) * n)
         self.bounds = torch.searchsorted(ukeys, q.reshape(-1).contiguous(), side="left")
-        del ukeys, slice_of_compact
+        del ukeys
         # entry tiles over the compact rows, dealt to XCD groups by slice
         T = schedule.tile_nnz if (schedule.tile_nnz > 0 and schedule.tile_mode == 1) else (
             4096 if data.dtype == torch.float32 else 2048)

@@ -491,13 +491,21 @@ def flagsparse_spmv_csr(data=None, indices=None, indptr=None, x=None, shape=None
             raise ValueError("out shape/dtype must match result")
     if float(beta) != 0.0 and out is None:
         raise ValueError("beta != 0 requires out (the y that is scaled and accumulated)")
-    direct = (out is not None and out.is_contiguous() and (alpha != 1.0 or beta != 0.0))
+    # with a contiguous `out` the kernels write it directly (the reference
+    # computes a new y and copies it into `out`: same values, one pass fewer)
+    direct = out is not None and out.is_contiguous()
     t0 = None
     if return_time:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
     if direct:
-        y = _direct_spmv(prepared, x, out, use_opt, alpha, beta)
+        try:
+            y = _direct_spmv(prepared, x, out, use_opt, alpha, beta)
+        except Exception as exc:
+            if prepared.index_fallback_policy != "auto" or not _spmv_uses_int64_indices(prepared):
+                raise
+            fb = _spmv_prepared_with_int32_indices(prepared, exc)
+            y = _direct_spmv(fb, x, out, use_opt, alpha, beta)
     else:
         y = _run_spmv_prepared_with_fallback(prepared, x, use_opt=use_opt)
     elapsed_ms = None
