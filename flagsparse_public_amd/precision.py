@@ -10,8 +10,11 @@
                          the row's max |v| (11 bits; 2 bytes + 4 per row).
 
 ``flagsparse_spmv_csr_split`` then computes y = alpha A x + beta y with fp32
-x / y and fp32 accumulation (compensated TwoSum summation for rows in the
-chunked long-row role, which holds every row over 4096 entries).  The
+x / y and fp32 products.  Row sums: fp64 in the tile roles (the default entry
+tiles), compensated TwoSum fp32 summation in the chunked long-row role of the
+row-group schedules.  When x is larger than 4 MB the default schedule also
+uses the column-sliced layout (slicing.py) with the planes permuted into
+slice-major order.  The
 reference stores values only in the call's dtype; these modes have no
 counterpart there and are measured beside the plain fp32 kernel.
 """
@@ -24,6 +27,7 @@ import torch
 
 from . import _lib
 from .rowsplit import RowSplitPlan, Schedule, default_spmv_schedule
+from .slicing import SlicedCsr
 from .sparse_operations.spmv_csr import _prepare_spmv_csr_matrix, _validate_spmv_x
 
 FORMATS = {
@@ -93,9 +97,25 @@ def prepare_spmv_csr_split(data, indices, indptr, shape, value_format="bf16x2", 
              if value_format == "f16_rowscale" else None)
     csr = _lib.make_csr(data, kernel_indices, kernel_indptr, n_rows, n_cols)
     _lib.split_values(fmt, csr, hi, lo, scale, dev)
-    sch = schedule or default_spmv_schedule(n_rows, nnz)
+    sch = schedule or default_spmv_schedule(n_rows, nnz, torch.float32, n_cols)
     return PreparedCsrSpmvSplit(kernel_indices, kernel_indptr, (n_rows, n_cols), value_format,
                                 hi, lo, scale, row_lengths, max_row_nnz, sch)
+
+
+def _sliced_split(prepared, sch):
+    """Slice-major copy of the encoded planes (see slicing.py); the row
+    scales of f16_rowscale follow each compact row's original row, so the
+    stored values are exactly those of the unsliced planes."""
+    key = ("sliced", sch.key())
+    sl = prepared._plans.get(key)
+    if sl is None:
+        sl = SlicedCsr(None, prepared.kernel_indices, prepared.kernel_indptr, prepared.n_rows,
+                       prepared.n_cols, sch.x_slice_cols, sch, _lib.FS_ACC_NATIVE,
+                       planes=(prepared.hi, prepared.lo), value_dtype=torch.float32)
+        sl.row_scale = (prepared.row_scale[sl.compact_row.to(torch.int64)].contiguous()
+                        if prepared.row_scale is not None else None)
+        prepared._plans[key] = sl
+    return sl
 
 
 def decoded_values(prepared) -> torch.Tensor:
@@ -130,8 +150,15 @@ def flagsparse_spmv_csr_split(x, prepared, out=None, alpha=1.0, beta=0.0, return
     if return_time:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-    if prepared.n_rows:
-        plan = prepared.plan(schedule)
+    sch = schedule or prepared.schedule
+    if prepared.n_rows and sch.x_slice_cols > 0 and prepared.n_cols > sch.x_slice_cols:
+        sl = _sliced_split(prepared, sch)
+        if sl.n_compact:
+            _lib.spmv_split(sl.csr, FORMATS[prepared.value_format], sl.planes[0], sl.planes[1],
+                            sl.row_scale, x, sl.z, 1.0, 0.0, sl.plan.struct(1), x.device)
+        _lib.sliced_combine(sl.desc, sl.z, y, alpha, beta, x.device)
+    elif prepared.n_rows:
+        plan = prepared.plan(sch)
         _lib.spmv_split(prepared._csr, FORMATS[prepared.value_format], prepared.hi, prepared.lo,
                         prepared.row_scale, x, y, alpha, beta, plan.struct(1), x.device)
     if return_time:

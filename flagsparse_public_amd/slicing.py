@@ -43,17 +43,21 @@ class SlicedCsr:
     """Slice-major copy of one CSR matrix plus its compact-row plan."""
 
     def __init__(self, data, indices, indptr, n_rows, n_cols, slice_cols, schedule,
-                 accumulate):
-        if data.dtype not in SLICED_DTYPES:
-            raise TypeError(f"column slicing supports {SLICED_DTYPES}, got {data.dtype}")
-        dev = data.device
+                 accumulate, planes=(), value_dtype=None):
+        """``data`` may be None when the values live in other per-entry
+        ``planes`` (split-precision storage, value_dtype float32); every plane
+        is permuted into slice-major order with the entries."""
+        value_dtype = data.dtype if data is not None else value_dtype
+        if value_dtype not in SLICED_DTYPES:
+            raise TypeError(f"column slicing supports {SLICED_DTYPES}, got {value_dtype}")
+        dev = indices.device
         self.n_rows, self.n_cols = int(n_rows), int(n_cols)
         n = self.n_rows
         P = slice_count(self.n_cols, slice_cols)
         self.n_slices = P
         self.width = max(1, math.ceil(self.n_cols / P))
         w = self.width
-        nnz = data.numel()
+        nnz = indices.numel()
         lens = (indptr[1:] - indptr[:-1]).to(torch.int64)
         rows = torch.repeat_interleave(torch.arange(n, device=dev, dtype=torch.int64), lens)
         sl = indices.to(torch.int64) // w
@@ -62,7 +66,8 @@ class SlicedCsr:
         order = torch.argsort(key, stable=True)
         key = key[order]
         self.cols = indices[order].contiguous()
-        self.vals = data[order].contiguous()
+        self.vals = data[order].contiguous() if data is not None else None
+        self.planes = [pl[order].contiguous() if pl is not None else None for pl in planes]
         del order
         ukeys, counts = torch.unique_consecutive(key, return_counts=True)
         del key
@@ -84,10 +89,9 @@ class SlicedCsr:
         self.bounds = torch.searchsorted(ukeys, q.reshape(-1).contiguous(), side="left")
         del ukeys
         # entry tiles over the compact rows, dealt to XCD groups by slice
-        T = schedule.tile_nnz if (schedule.tile_nnz > 0 and schedule.tile_mode == 1) else (
-            4096 if data.dtype == torch.float32 else 2048)
+        T = schedule.tile_nnz if (schedule.tile_nnz > 0 and schedule.tile_mode == 1) else 2048
         self.schedule = Schedule(tile_nnz=T, tile_mode=1)
-        self.plan = RowSplitPlan(self.cptr, self.n_compact, data.dtype, self.schedule, accumulate)
+        self.plan = RowSplitPlan(self.cptr, self.n_compact, value_dtype, self.schedule, accumulate)
         qn = P // N_GROUPS
         first = slice_entry[torch.arange(0, P, qn, device=dev)] // T
         gt = torch.empty(N_GROUPS + 1, dtype=torch.int64, device=dev)
@@ -97,9 +101,14 @@ class SlicedCsr:
         self.group_tiles = gt
         self.group_blocks = int((gt[1:] - gt[:-1]).max().item())
         self.plan.set_groups(N_GROUPS, self.group_blocks, self.group_tiles)
-        self.csr = _lib.make_csr(self.vals, self.cols, self.cptr, self.n_compact, self.n_cols)
-        self.z = torch.empty(max(self.n_compact, 1), dtype=data.dtype, device=dev)
-        self.desc = _lib.FsSlices(value_type=_lib.value_code(data.dtype), n_slices=P,
+        self.csr = _lib.FsCsr(value_type=_lib.value_code(value_dtype),
+                              index_type=_lib.index_code(self.cols.dtype),
+                              ptr_type=_lib.index_code(self.cptr.dtype), reserved=0,
+                              n_rows=self.n_compact, n_cols=self.n_cols, nnz=nnz,
+                              row_ptr=_lib.ptr(self.cptr), col_idx=_lib.ptr(self.cols),
+                              values=_lib.ptr(self.vals))
+        self.z = torch.empty(max(self.n_compact, 1), dtype=value_dtype, device=dev)
+        self.desc = _lib.FsSlices(value_type=_lib.value_code(value_dtype), n_slices=P,
                                   n_rows=n, n_compact=self.n_compact,
                                   compact_row=_lib.ptr(self.compact_row),
                                   bounds=_lib.ptr(self.bounds), rows_per_block=COMBINE_ROWS)

@@ -41,9 +41,15 @@ def _log(msg):
     print(f"[train_selector] {msg}", flush=True)
 
 
-def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=100):
+def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=100,
+            deadline=None):
+    """Time every variant on each matrix; returns False when the deadline
+    (time.time() value) stopped collection early."""
     dense_cache = {}
     for k, seed in enumerate(seeds):
+        if deadline is not None and time.time() > deadline:
+            _log(f"{label}: time budget reached after {k} matrices")
+            return False
         family, n = synthetic.selector_matrix_spec(seed)
         n = min(n, max_n)
         indptr, cols, vals, params = synthetic.family_csr_torch(family, n, seed, device)
@@ -69,6 +75,7 @@ def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=10
             _log(f"{label} {k + 1}/{len(seeds)} matrices, {time.time() - t0:.0f}s "
                  f"(last: {family} n={n} nnz={nnz}, spmv best {tv.min():.1f}us, "
                  f"spmm best {tm.min():.1f}us)")
+    return True
 
 
 def evaluate(sel: Selector, db: TimingDB, default_index: int, device, heuristic=None):
@@ -121,6 +128,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=4000)
     ap.add_argument("--resume", action="store_true", help="reuse DIR/*_db.npz if present")
     ap.add_argument("--collect-only", action="store_true")
+    ap.add_argument("--time-budget", type=float, default=0.0,
+                    help="stop collecting after this many seconds (saved; resume later)")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     device = torch.device("cuda", 0)
@@ -129,7 +138,10 @@ def main(argv=None):
     paths = {s: {op: os.path.join(args.out, f"{op}_{s}_db.npz") for op in ("spmv", "spmm64")}
              for s in ("train", "test")}
     dbs = {}
-    for split, seed0, count in (("train", TRAIN_SEED0, args.n_train), ("test", TEST_SEED0, args.n_test)):
+    deadline = t0 + args.time_budget if args.time_budget > 0 else None
+    complete = True
+    # held-out matrices first, so a partial collection can already be judged
+    for split, seed0, count in (("test", TEST_SEED0, args.n_test), ("train", TRAIN_SEED0, args.n_train)):
         have = {}
         if args.resume and all(os.path.exists(p) for p in paths[split].values()):
             have = {op: TimingDB.load(p) for op, p in paths[split].items()}
@@ -141,11 +153,13 @@ def main(argv=None):
         def save(have=have, split=split):
             for op, db in have.items():
                 db.save(paths[split][op])
-        if seeds:
-            collect(seeds, device, have, args.max_n, args.reps, t0, split, save=save)
+        if seeds and complete:
+            complete = collect(seeds, device, have, args.max_n, args.reps, t0, split, save=save,
+                               deadline=deadline)
             save()
         dbs[split] = have
-    if args.collect_only:
+    if args.collect_only or not complete:
+        _log(f"collected: test {len(dbs['test']['spmv'])}, train {len(dbs['train']['spmv'])}")
         return 0
     report = {"device": name, "n_train": len(dbs["train"]["spmv"]),
               "n_test": len(dbs["test"]["spmv"])}

@@ -64,8 +64,10 @@ def test_f16_rowscale_encoding():
 
 @pytest.mark.parametrize("fmt", ["bf16x2", "bf16", "f16_rowscale"])
 @pytest.mark.parametrize("sch", [None, Schedule(group_lanes=4, short_max=64, chunk_nnz=256),
-                                 Schedule(group_lanes=32, short_max=4096, chunk_nnz=4096, xcd_remap=1)],
-                         ids=["default", "s4", "s32"])
+                                 Schedule(group_lanes=32, short_max=4096, chunk_nnz=4096, xcd_remap=1),
+                                 Schedule(tile_nnz=2048, tile_mode=1),
+                                 Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=1500)],
+                         ids=["default", "s4", "s32", "entry", "sliced"])
 def test_split_spmv_parity_and_stated_error(orc, fmt, sch):
     ip, cols, vals = _matrix()
     n = ip.size - 1
