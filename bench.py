@@ -169,14 +169,15 @@ def main(argv=None):
     # the dominant kernel alone: the compact-row SpMV of the sliced layout, or
     # the whole SpMV when the matrix is not sliced
     from flagsparse_public_amd import _lib
-    sliced = prepared._plans.get(("sliced", sch.key())) if use_opt else None
+    sliced = prepared._plans.get(("sliced", sch.key(), _lib.FS_ACC_NATIVE if use_opt
+                                  else _lib.FS_ACC_WIDE))
     if sliced is not None:
         kplan = sliced.plan.struct(1)
 
         def kernel():
-            _lib.spmv(sliced.csr, d_x, sliced.z, 1.0, 0.0, kplan, dev)
+            _lib.spmv(sliced.csr, d_x, None if sliced.sums else sliced.z, 1.0, 0.0, kplan, dev)
         nc = sliced.n_compact
-        kbytes = (nc + 1) * 4 + nnz * 12 + nc * 4
+        kbytes = (nc + 1) * 4 + nnz * 12 + nc * sliced.z.element_size()
         kname = "spmv_entry_kernel (compact rows) + carry fix-up"
     else:
         kernel = step

@@ -81,6 +81,7 @@ class FsPlan(ctypes.Structure):
         ("n_groups", ctypes.c_int32),
         ("group_blocks", ctypes.c_int64),
         ("group_tiles", ctypes.c_void_p),
+        ("row_sums", ctypes.c_void_p),
     ]
 
 
@@ -108,7 +109,7 @@ class FsSlices(ctypes.Structure):
         ("compact_row", ctypes.c_void_p),
         ("bounds", ctypes.c_void_p),
         ("rows_per_block", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("z_type", ctypes.c_int32),
     ]
 
 

@@ -55,8 +55,16 @@ LINK_FLAGS = [
 ]
 
 
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(PKG_DIR, '..', 'include')}"]
+# host-only C++ sources of the main library (vendor_rocsparse.cpp builds its own)
+HOST_SOURCES = ("mtx_io.cpp",)
+
+
 def _compile(src: str, obj: str) -> tuple[str, int, str]:
-    cmd = [HIPCC, *COMPILE_FLAGS, "-c", src, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = ["g++", *HOST_FLAGS, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *COMPILE_FLAGS, "-c", src, "-o", obj]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     return src, proc.returncode, proc.stdout + proc.stderr
 
@@ -64,14 +72,15 @@ def _compile(src: str, obj: str) -> tuple[str, int, str]:
 def build(jobs: int | None = None, force: bool = True, verbose: bool = False) -> str:
     """Compile every HIP source and link the shared library; returns its path."""
     os.makedirs(BUILD_DIR, exist_ok=True)
-    sources = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    sources = sorted(glob.glob(os.path.join(CSRC, "*.hip"))) + [
+        os.path.join(CSRC, f) for f in HOST_SOURCES]
     headers = sorted(glob.glob(os.path.join(CSRC, "*.hpp"))) + [
         os.path.join(PKG_DIR, "..", "include", "flagsparse_hip.h")]
     newest_header = max(os.path.getmtime(h) for h in headers)
     todo = []
     objs = []
     for src in sources:
-        obj = os.path.join(BUILD_DIR, os.path.basename(src)[:-4] + ".o")
+        obj = os.path.join(BUILD_DIR, os.path.splitext(os.path.basename(src))[0] + ".o")
         objs.append(obj)
         stale = (force or not os.path.exists(obj)
                  or os.path.getmtime(obj) < max(os.path.getmtime(src), newest_header))

@@ -201,7 +201,7 @@ extern "C" int fs_spmv_csr_split(const fs_csr* A, int32_t format, const void* hi
   rc = check_plan(plan, A, 1, "fs_spmv_csr_split");
   if (rc) return rc;
   if (A->n_rows == 0) return FS_OK;
-  if (!y) return fail(FS_ERR_VALUE, "fs_spmv_csr_split: y is null");
+  if (!y && !plan->row_sums) return fail(FS_ERR_VALUE, "fs_spmv_csr_split: y is null");
   if (A->nnz > 0 && (!x || !hi)) return fail(FS_ERR_VALUE, "fs_spmv_csr_split: x or values are null");
   hipStream_t st = as_stream(stream);
   switch (format) {

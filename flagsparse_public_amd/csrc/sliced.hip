@@ -26,12 +26,12 @@ constexpr int kCombineRows = 256;  // rows per block (one per thread)
 // then added to the LDS row sums one slice at a time.
 constexpr int kGroup = 8;
 
-template <typename V>
+template <typename Z, typename V>
 __global__ __launch_bounds__(kCombineRows) void sliced_combine_kernel(
-    const V* __restrict__ z, const int32_t* __restrict__ compact_row,
+    const Z* __restrict__ z, const int32_t* __restrict__ compact_row,
     const int64_t* __restrict__ bounds, int32_t n_slices, int64_t n_rows, int64_t n_blocks,
     V* __restrict__ y, double alpha, double beta) {
-  using C = typename Native<V>::T;
+  using C = typename Native<Z>::T;
   using S = typename TileSum<C>::T;
   __shared__ S acc[kCombineRows];
   const int64_t b = blockIdx.x;
@@ -68,13 +68,13 @@ __global__ __launch_bounds__(kCombineRows) void sliced_combine_kernel(
   if (row < n_rows) write_out(y, row, acc[tid], alpha, beta);
 }
 
-template <typename V>
+template <typename Z, typename V>
 int launch_combine(const fs_slices* sl, const void* z, void* y, double alpha, double beta,
                    hipStream_t st) {
   const int64_t nb = (sl->n_rows + kCombineRows - 1) / kCombineRows;
   if (nb == 0) return FS_OK;
-  sliced_combine_kernel<V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
-      static_cast<const V*>(z), sl->compact_row, sl->bounds, sl->n_slices, sl->n_rows, nb,
+  sliced_combine_kernel<Z, V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
+      static_cast<const Z*>(z), sl->compact_row, sl->bounds, sl->n_slices, sl->n_rows, nb,
       static_cast<V*>(y), alpha, beta);
   return check_launch("fs_sliced_combine");
 }
@@ -94,11 +94,25 @@ extern "C" int fs_sliced_combine(const fs_slices* sl, const void* z, void* y, do
   if (!y || !sl->bounds || (sl->n_compact > 0 && (!z || !sl->compact_row)))
     return fail(FS_ERR_VALUE, "fs_sliced_combine: null pointer");
   hipStream_t st = as_stream(stream);
+  const bool sums = sl->z_type != sl->value_type;
   switch (sl->value_type) {
-    case FS_F32: return launch_combine<float>(sl, z, y, alpha, beta, st);
-    case FS_F64: return launch_combine<double>(sl, z, y, alpha, beta, st);
-    case FS_C64: return launch_combine<c64_t>(sl, z, y, alpha, beta, st);
-    case FS_C128: return launch_combine<c128_t>(sl, z, y, alpha, beta, st);
-    default: return fail(FS_ERR_TYPE, "fs_sliced_combine: value_type %d is not sliced", sl->value_type);
+    case FS_F32:
+      if (!sums) return launch_combine<float, float>(sl, z, y, alpha, beta, st);
+      if (sl->z_type == FS_F64) return launch_combine<double, float>(sl, z, y, alpha, beta, st);
+      break;
+    case FS_F64:
+      if (!sums) return launch_combine<double, double>(sl, z, y, alpha, beta, st);
+      break;
+    case FS_C64:
+      if (!sums) return launch_combine<c64_t, c64_t>(sl, z, y, alpha, beta, st);
+      if (sl->z_type == FS_C128) return launch_combine<c128_t, c64_t>(sl, z, y, alpha, beta, st);
+      break;
+    case FS_C128:
+      if (!sums) return launch_combine<c128_t, c128_t>(sl, z, y, alpha, beta, st);
+      break;
+    default:
+      return fail(FS_ERR_TYPE, "fs_sliced_combine: value_type %d is not sliced", sl->value_type);
   }
+  return fail(FS_ERR_TYPE, "fs_sliced_combine: partials of type %d cannot feed value_type %d",
+              sl->z_type, sl->value_type);
 }

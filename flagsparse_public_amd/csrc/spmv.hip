@@ -127,7 +127,7 @@ extern "C" int fs_spmv_csr(const fs_csr* A, const void* x, void* y, double alpha
   rc = check_plan(plan, A, 1, "fs_spmv_csr");
   if (rc) return rc;
   if (A->n_rows == 0) return FS_OK;
-  if (!y) return fail(FS_ERR_VALUE, "fs_spmv_csr: y is null");
+  if (!y && !plan->row_sums) return fail(FS_ERR_VALUE, "fs_spmv_csr: y is null");
   if (A->n_cols > 0 && !x) return fail(FS_ERR_VALUE, "fs_spmv_csr: x is null");
   hipStream_t st = as_stream(stream);
   const bool wide = plan->accumulate == FS_ACC_WIDE;

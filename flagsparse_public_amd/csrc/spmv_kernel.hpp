@@ -84,6 +84,7 @@ struct SpmvParams {
   int64_t nnz;
   int n_groups;                // entry tiles: block groups (0: block b = tile b)
   const int64_t* group_tiles;  // [n_groups + 1]
+  void* row_sums;              // entry tiles: raw row sums instead of y (or null)
 };
 
 // Accumulator used for chunks: fp32-native chunked rows get compensation.
@@ -337,6 +338,7 @@ int64_t spmv_fill_schedule(SpmvParams<VS, V, I, P>& p, const fs_csr* M, const fs
   p.nnz = M->nnz;
   p.n_groups = pl->n_groups;
   p.group_tiles = pl->group_tiles;
+  p.row_sums = pl->row_sums;
   if (pl->tile_nnz > 0 && pl->tile_mode == FS_TILE_ENTRIES) {
     if (pl->n_groups <= 0) return pl->n_tiles;
     return pl->group_blocks * pl->n_groups;
@@ -377,11 +379,24 @@ template <typename C> __host__ __device__ inline size_t entry_lds_bytes(int64_t 
          sizeof(S) * kWavesPerBlock;
 }
 
+// A finished row of the entry role: y = alpha * sum + beta * y, or, with
+// row_sums set, the raw sum in the sum type (times the row scale of scaled
+// storage) for a later pass to combine.
+template <typename VS, typename V, typename I, typename P, typename S>
+__device__ __forceinline__ void entry_write(const SpmvParams<VS, V, I, P>& p, int64_t row, S acc) {
+  if (p.row_sums) {
+    if constexpr (VS::kRowScale) acc = scale(acc, (double)p.vals.row_scale(row));
+    static_cast<S*>(p.row_sums)[row] = acc;
+  } else {
+    write_row(p.vals, p.y, row, acc, p.alpha, p.beta);
+  }
+}
+
 template <typename A, typename VS, typename V, typename I, typename P, typename S>
 __device__ __forceinline__ void entry_finish(const SpmvParams<VS, V, I, P>& p, int64_t row,
                                              S acc, bool to_carry, S* carries, int64_t t) {
   if (to_carry) carries[2 * t + 1] = acc;
-  else write_row(p.vals, p.y, row, acc, p.alpha, p.beta);
+  else entry_write(p, row, acc);
 }
 
 template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>
@@ -549,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS,
     acc = add(acc, carries[2 * u]);
     if (p.tile_start[u + 1] > p.tile_start[u]) break;
   }
-  write_row(p.vals, p.y, r1 - 1, acc, p.alpha, p.beta);
+  entry_write(p, r1 - 1, acc);
 }
 
 template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>

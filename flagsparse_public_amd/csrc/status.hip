@@ -79,6 +79,9 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
     if (!P->tile_start) return fail(FS_ERR_VALUE, "%s: entry tiles need tile_start", who);
     if (P->n_groups < 0 || (P->n_groups > 0 && (!P->group_tiles || P->group_blocks < 0)))
       return fail(FS_ERR_VALUE, "%s: bad block groups", who);
+    if (A->value_type != FS_F32 && A->value_type != FS_F64 && A->value_type != FS_C64 &&
+        A->value_type != FS_C128 && P->row_sums)
+      return fail(FS_ERR_VALUE, "%s: row sums are offered for fp32 / fp64 / complex values", who);
     const int64_t need = fs_tile_carry_bytes(A->value_type, P->n_tiles);
     if (!P->partials || P->partials_bytes < need)
       return fail(FS_ERR_VALUE, "%s: carry buffer too small (%lld < %lld bytes)", who,

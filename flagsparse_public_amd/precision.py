@@ -106,7 +106,7 @@ def _sliced_split(prepared, sch):
     """Slice-major copy of the encoded planes (see slicing.py); the row
     scales of f16_rowscale follow each compact row's original row, so the
     stored values are exactly those of the unsliced planes."""
-    key = ("sliced", sch.key())
+    key = ("sliced", sch.key(), _lib.FS_ACC_NATIVE)
     sl = prepared._plans.get(key)
     if sl is None:
         sl = SlicedCsr(None, prepared.kernel_indices, prepared.kernel_indptr, prepared.n_rows,

@@ -132,6 +132,7 @@ class RowSplitPlan:
         self._work = {}
         self._structs = {}
         self.n_groups, self.group_blocks, self.group_tiles = 0, 0, None
+        self.row_sums = None
         if schedule.tile_nnz > 0 and schedule.tile_mode == _lib.FS_TILE_ENTRIES:
             self._init_entry_tiles(indptr, n_rows, schedule)
             return
@@ -193,6 +194,11 @@ class RowSplitPlan:
         self.group_tiles = group_tiles
         self._structs.clear()
 
+    def set_row_sums(self, sums: torch.Tensor):
+        """Entry tiles: store raw fp64 row sums here instead of writing y."""
+        self.row_sums = sums
+        self._structs.clear()
+
     def struct(self, n_dense: int = 1) -> _lib.FsPlan:
         """fs_plan for a call with n_dense dense columns (1 for SpMV)."""
         key = max(int(n_dense), 1)
@@ -215,7 +221,8 @@ class RowSplitPlan:
                              n_tiles=self.n_tiles, tile_start=_lib.ptr(self.tile_start),
                              tile_mode=_lib.FS_TILE_ENTRIES, n_groups=self.n_groups,
                              group_blocks=self.group_blocks,
-                             group_tiles=_lib.ptr(self.group_tiles))
+                             group_tiles=_lib.ptr(self.group_tiles),
+                             row_sums=_lib.ptr(self.row_sums))
             self._structs[key] = st
             return st
         if self.n_long:

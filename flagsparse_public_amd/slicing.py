@@ -43,10 +43,13 @@ class SlicedCsr:
     """Slice-major copy of one CSR matrix plus its compact-row plan."""
 
     def __init__(self, data, indices, indptr, n_rows, n_cols, slice_cols, schedule,
-                 accumulate, planes=(), value_dtype=None):
+                 accumulate, planes=(), value_dtype=None, sums=False):
         """``data`` may be None when the values live in other per-entry
         ``planes`` (split-precision storage, value_dtype float32); every plane
-        is permuted into slice-major order with the entries."""
+        is permuted into slice-major order with the entries.  ``sums``: the
+        partials are the kernels' raw fp64 row sums (complex: two fp64)
+        instead of values rounded to the value dtype -- the default fp32 path,
+        which accumulates in fp64 and rounds once."""
         value_dtype = data.dtype if data is not None else value_dtype
         if value_dtype not in SLICED_DTYPES:
             raise TypeError(f"column slicing supports {SLICED_DTYPES}, got {value_dtype}")
@@ -107,14 +110,22 @@ class SlicedCsr:
                               n_rows=self.n_compact, n_cols=self.n_cols, nnz=nnz,
                               row_ptr=_lib.ptr(self.cptr), col_idx=_lib.ptr(self.cols),
                               values=_lib.ptr(self.vals))
-        self.z = torch.empty(max(self.n_compact, 1), dtype=value_dtype, device=dev)
+        self.sums = bool(sums)
+        z_dtype = value_dtype
+        if self.sums:
+            z_dtype = torch.complex128 if value_dtype.is_complex else torch.float64
+        self.z = torch.empty(max(self.n_compact, 1), dtype=z_dtype, device=dev)
+        if self.sums:
+            self.plan.set_row_sums(self.z)
         self.desc = _lib.FsSlices(value_type=_lib.value_code(value_dtype), n_slices=P,
                                   n_rows=n, n_compact=self.n_compact,
                                   compact_row=_lib.ptr(self.compact_row),
-                                  bounds=_lib.ptr(self.bounds), rows_per_block=COMBINE_ROWS)
+                                  bounds=_lib.ptr(self.bounds), rows_per_block=COMBINE_ROWS,
+                                  z_type=_lib.value_code(z_dtype))
 
     def run(self, x, y, alpha, beta, device):
         """y = alpha A x + beta y: compact-row partials, then the combine."""
         if self.n_compact > 0:
-            _lib.spmv(self.csr, x, self.z, 1.0, 0.0, self.plan.struct(1), device)
+            _lib.spmv(self.csr, x, None if self.sums else self.z, 1.0, 0.0, self.plan.struct(1),
+                      device)
         _lib.sliced_combine(self.desc, self.z, y, alpha, beta, device)

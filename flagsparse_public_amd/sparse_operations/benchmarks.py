@@ -60,9 +60,8 @@ def _vendor_op_matrix(data, indices, indptr, shape, op_code):
         return data, indices, indptr, shape
     from .spmv_csr import _transpose_csr_for_spmv
     d = data.conj() if (op_code == 2 and data.is_complex()) else data
-    td, ti, tp = _transpose_csr_for_spmv(d.resolve_conj() if d.is_complex() else d, indices,
-                                         indptr, shape)
-    return td, ti, tp, (int(shape[1]), int(shape[0]))
+    return _transpose_csr_for_spmv(d.resolve_conj() if d.is_complex() else d, indices, indptr,
+                                   shape)
 
 
 def benchmark_spmv_case(n_rows=4096, n_cols=4096, nnz=65536, value_dtype=torch.float32,

@@ -360,13 +360,15 @@ def _launch_spmv(prepared, x, accumulate, out=None, alpha=1.0, beta=0.0, schedul
         return y
     sch = schedule or prepared.schedule
     if (sch is not None and sch.x_slice_cols > 0 and prepared.n_cols > sch.x_slice_cols
-            and accumulate == _lib.FS_ACC_NATIVE and prepared.data.dtype in SLICED_DTYPES):
-        key = ("sliced", sch.key())
+            and prepared.data.dtype in SLICED_DTYPES):
+        # native partials in the value dtype; the default fp32 path (fp64
+        # accumulation, one rounding) keeps raw fp64 row sums as partials
+        key = ("sliced", sch.key(), int(accumulate))
         sliced = prepared._plans.get(key)
         if sliced is None:
             sliced = SlicedCsr(prepared.data, prepared.kernel_indices, prepared.kernel_indptr,
                                prepared.n_rows, prepared.n_cols, sch.x_slice_cols, sch,
-                               accumulate)
+                               accumulate, sums=accumulate == _lib.FS_ACC_WIDE)
             prepared._plans[key] = sliced
         sliced.run(x, y, alpha, beta, prepared.data.device)
         return y

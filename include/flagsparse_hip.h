@@ -122,6 +122,7 @@ typedef struct fs_plan {
   int32_t n_groups;              /* FS_TILE_ENTRIES: 0, or block groups    */
   int64_t group_blocks;          /* blocks per group (largest tile range)  */
   const int64_t *group_tiles;    /* [n_groups + 1] tile range of a group   */
+  void *row_sums;                /* FS_TILE_ENTRIES: if set, raw row sums  */
 } fs_plan;
 
 /*
@@ -144,11 +145,43 @@ typedef struct fs_plan {
  *                    8 XCDs round-robin, so G = 8 keeps each XCD on its own
  *                    contiguous tile range -- and its L2 on its own x columns
  *                    when the matrix is stored column-slice-major.
+ *                    With row_sums set, y / alpha / beta are not used: row r's
+ *                    sum is stored to row_sums[r] in the sum type (fp64; two
+ *                    fp64 for complex), times its row scale for
+ *                    FS_STORE_F16_ROWSCALE, for fs_sliced_combine to add up.
  */
 enum { FS_TILE_ROWS = 0, FS_TILE_ENTRIES = 1 };
 
 /* Bytes of `partials` for FS_TILE_ENTRIES with n_tiles tiles. */
 int64_t fs_tile_carry_bytes(int32_t value_type, int64_t n_tiles);
+
+/* ------------------------------------------------ MatrixMarket input */
+/*
+ * Replaces the reference harness's Python loader load_mtx_to_csr_torch
+ * (tests/test_spmv.py:103-176) with its reading rules: banner field /
+ * symmetry, '%' comments, the first `entries` data lines, pattern = 1.0,
+ * complex files read as their real part, entries outside the matrix dropped,
+ * symmetric / hermitian mirrored, skew-symmetric mirrored negated, repeated
+ * pairs summed in fp64 in file order, columns ascending per row.
+ * fs_mtx_read parses into host memory and reports the CSR sizes;
+ * fs_mtx_fill copies row_ptr[n_rows + 1], col_idx[nnz] (int64) and
+ * values[nnz] (fp64) out (any of them may be null); fs_mtx_free releases.
+ */
+enum { FS_MTX_REAL = 0, FS_MTX_INTEGER = 1, FS_MTX_COMPLEX = 2, FS_MTX_PATTERN = 3 };
+enum { FS_MTX_GENERAL = 0, FS_MTX_SYMMETRIC = 1, FS_MTX_SKEW = 2, FS_MTX_HERMITIAN = 3 };
+
+typedef struct fs_mtx_info {
+  int64_t n_rows;
+  int64_t n_cols;
+  int64_t nnz;        /* after mirroring and summing repeated pairs */
+  int32_t field;      /* FS_MTX_REAL ... FS_MTX_PATTERN            */
+  int32_t symmetry;   /* FS_MTX_GENERAL ... FS_MTX_HERMITIAN       */
+} fs_mtx_info;
+
+int fs_mtx_read(const char *path, fs_mtx_info *info, void **handle);
+int fs_mtx_fill(void *handle, int64_t *row_ptr, int64_t *col_idx, double *values);
+void fs_mtx_free(void *handle);
+const char *fs_mtx_last_error(void);
 
 /* ------------------------------------------------- column-sliced SpMV */
 /*
@@ -157,12 +190,13 @@ int64_t fs_tile_carry_bytes(int32_t value_type, int64_t n_tiles);
  * is larger than one XCD's L2.  The matrix is stored column-slice-major:
  * every (slice, row) pair with entries is a compact row, ordered by slice,
  * then row.  fs_spmv_csr over the compact rows (absolute column indices)
- * writes one partial per compact row into z; fs_sliced_combine then forms
+ * writes one partial per compact row into z (as y, or as raw fp64 row sums
+ * through fs_plan.row_sums); fs_sliced_combine then forms
  * y[r] = alpha * sum over slices of z[compact row (k, r)] + beta * y[r],
  * summed in slice order in fp64.
  */
 typedef struct fs_slices {
-  int32_t value_type;        /* FS_F32, FS_F64, FS_C64 or FS_C128          */
+  int32_t value_type;        /* of y: FS_F32, FS_F64, FS_C64 or FS_C128    */
   int32_t n_slices;
   int64_t n_rows;            /* rows of the original matrix                */
   int64_t n_compact;
@@ -171,7 +205,8 @@ typedef struct fs_slices {
                                 ceil(n_rows / rows_per_block): first compact
                                 row of slice k in row block b               */
   int32_t rows_per_block;    /* 256                                        */
-  int32_t pad;
+  int32_t z_type;            /* of the partials: value_type, or FS_F64 /
+                                FS_C128 for row sums (fs_plan.row_sums)     */
 } fs_slices;
 
 int fs_sliced_combine(const fs_slices *sl, const void *z, void *y, double alpha, double beta,

@@ -1,0 +1,65 @@
+"""The benchmark harness entry points (benchmarks.py:345-507, spmm_csr.py:2226-2593,
+spmm_csr_opt_alg2.py:862-987): result layout, verification against torch
+sparse, and the rocSPARSE vendor column in the reference's cusparse_* fields."""
+
+import pytest
+import torch
+
+import flagsparse_public_amd as fs
+from flagsparse_public_amd import vendor
+
+pytestmark = pytest.mark.gpu
+
+KEYS = {"parameters", "performance", "verification", "backend_status", "samples"}
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.complex64, torch.float16])
+@pytest.mark.parametrize("op", ["non", "trans"])
+def test_benchmark_spmv_case(dtype, op):
+    r = fs.benchmark_spmv_case(n_rows=3000, n_cols=2000, nnz=40000, value_dtype=dtype,
+                               warmup=2, iters=5, op=op)
+    assert KEYS <= set(r)
+    assert r["verification"]["triton_match_reference"]
+    assert r["performance"]["triton_ms"] > 0
+    if dtype == torch.float16:
+        assert r["performance"]["cusparse_ms"] is None
+        assert "float16" in r["backend_status"]["cusparse_unavailable_reason"]
+    else:
+        assert r["backend_status"]["cusparse_unavailable_reason"] is None, r["backend_status"]
+        assert r["performance"]["cusparse_ms"] > 0
+        assert r["verification"]["cusparse_match_reference"]
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_benchmark_spmm_cases(dtype):
+    r = fs.benchmark_spmm_case(n_rows=2000, n_cols=1500, nnz=30000, n_dense_cols=16,
+                               value_dtype=dtype, warmup=2, iters=5)
+    assert KEYS <= set(r)
+    assert r["verification"]["triton_strict_allclose_match"]
+    assert r["verification"]["cusparse_strict_allclose_match"], r["backend_status"]
+    r2 = fs.comprehensive_spmm_test(n_rows=500, n_cols=700, nnz=5000, n_dense_cols=8,
+                                    dtype=dtype, warmup=1, iters=2)
+    assert r2["parameters"]["n_dense_cols"] == 8
+    r3 = fs.benchmark_spmm_opt_case(n_rows=2000, n_cols=1500, nnz=30000, n_dense_cols=32,
+                                    value_dtype=dtype, warmup=2, iters=5)
+    assert r3["verification"]["opt_ok"] and r3["performance"]["cu_ms"] > 0
+    r4 = fs.benchmark_spmm_opt_alg2_case(n_rows=2000, n_cols=1500, nnz=30000, n_dense_cols=32,
+                                         value_dtype=dtype, warmup=2, iters=5)
+    v = r4["verification"]
+    assert max(v["base_vs_torch_err"], v["opt_vs_torch_err"], v["opt_alg2_vs_torch_err"]) <= 1.0
+    assert v["opt_vs_cusparse_err"] <= 1.0
+
+
+def test_vendor_spmv_matches_library():
+    """rocSPARSE adaptive and this library agree on a power-law matrix."""
+    from flagsparse_public_amd import prepare_spmv_csr, flagsparse_spmv_csr, synthetic
+    ip, cols, vals = synthetic.powerlaw_csr(20000, 3, alpha=1.7, max_len=8000)
+    dev = "cuda"
+    d = [torch.from_numpy(a).to(dev) for a in (vals, cols, ip.astype("int32"))]
+    n = ip.size - 1
+    x = torch.rand(n, device=dev) * 2 - 1
+    y = torch.empty(n, device=dev)
+    v = vendor.VendorSpmv(*d, (n, n), x, y, "adaptive")
+    v.run()
+    ours = flagsparse_spmv_csr(x=x, prepared=prepare_spmv_csr(*d, (n, n)), use_opt=True)
+    assert torch.allclose(ours, y, rtol=1e-4, atol=1e-4)
