@@ -113,6 +113,16 @@ class FsSlices(ctypes.Structure):
     ]
 
 
+class FsMtxInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", ctypes.c_int64),
+        ("n_cols", ctypes.c_int64),
+        ("nnz", ctypes.c_int64),
+        ("field", ctypes.c_int32),
+        ("symmetry", ctypes.c_int32),
+    ]
+
+
 class FsMlp(ctypes.Structure):
     _fields_ = [
         ("in_dim", ctypes.c_int32),
@@ -167,6 +177,11 @@ _SIGNATURES = {
                                                             ctypes.c_double, ctypes.c_int64,
                                                             ctypes.c_void_p]),
     "fs_mlp_select": (ctypes.c_int, [ctypes.POINTER(FsMlp)] + [ctypes.c_void_p] * 6),
+    "fs_mtx_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FsMtxInfo),
+                                   ctypes.POINTER(ctypes.c_void_p)]),
+    "fs_mtx_fill": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "fs_mtx_free": (None, [ctypes.c_void_p]),
+    "fs_mtx_last_error": (ctypes.c_char_p, []),
     "fs_last_error": (ctypes.c_char_p, []),
     "fs_version": (ctypes.c_char_p, []),
 }

@@ -25,35 +25,10 @@ FIELDS = {0: "real", 1: "integer", 2: "complex", 3: "pattern"}
 SYMMETRIES = {0: "general", 1: "symmetric", 2: "skew-symmetric", 3: "hermitian"}
 
 
-class FsMtxInfo(ctypes.Structure):
-    _fields_ = [
-        ("n_rows", ctypes.c_int64),
-        ("n_cols", ctypes.c_int64),
-        ("nnz", ctypes.c_int64),
-        ("field", ctypes.c_int32),
-        ("symmetry", ctypes.c_int32),
-    ]
-
-
-def _mtx_lib():
-    lib = _lib.load()
-    if not getattr(lib, "_mtx_ready", False):
-        lib.fs_mtx_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(FsMtxInfo),
-                                    ctypes.POINTER(ctypes.c_void_p)]
-        lib.fs_mtx_read.restype = ctypes.c_int
-        lib.fs_mtx_fill.argtypes = [ctypes.c_void_p] * 4
-        lib.fs_mtx_fill.restype = ctypes.c_int
-        lib.fs_mtx_free.argtypes = [ctypes.c_void_p]
-        lib.fs_mtx_free.restype = None
-        lib.fs_mtx_last_error.restype = ctypes.c_char_p
-        lib._mtx_ready = True
-    return lib
-
-
 def read_mtx(file_path):
     """(indptr, indices, values fp64) as numpy arrays, shape, info dict."""
-    lib = _mtx_lib()
-    info = FsMtxInfo()
+    lib = _lib.load()
+    info = _lib.FsMtxInfo()
     handle = ctypes.c_void_p()
     rc = lib.fs_mtx_read(str(file_path).encode(), ctypes.byref(info), ctypes.byref(handle))
     if rc != 0:

@@ -33,7 +33,8 @@ def test_struct_layouts_match_header(tmp_path):
     """ctypes mirrors of fs_csr / fs_plan have the C compiler's offsets."""
     import subprocess
     classes = {"fs_csr": _lib.FsCsr, "fs_plan": _lib.FsPlan, "fs_batch_entry": _lib.FsBatchEntry,
-               "fs_mlp": _lib.FsMlp, "fs_slices": _lib.FsSlices}
+               "fs_mlp": _lib.FsMlp, "fs_slices": _lib.FsSlices,
+               "fs_mtx_info": _lib.FsMtxInfo}
     fields = {st: [f for f, _ in cls._fields_] for st, cls in classes.items()}
     fields["fs_batch_item"] = ["matrix", "kind", "row_begin", "row_end"]
     lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){"]

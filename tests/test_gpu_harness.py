@@ -63,3 +63,35 @@ def test_vendor_spmv_matches_library():
     v.run()
     ours = flagsparse_spmv_csr(x=x, prepared=prepare_spmv_csr(*d, (n, n)), use_opt=True)
     assert torch.allclose(ours, y, rtol=1e-4, atol=1e-4)
+
+
+def test_mtx_batch_harness(tmp_path):
+    """The SuiteSparse harness over MatrixMarket files (tests/test_spmv.py:416-665):
+    PASS rows for good files, an ERROR row for a broken one."""
+    from flagsparse_public_amd import mtx
+    import numpy as np
+    rng = np.random.default_rng(5)
+    paths = []
+    for k, sym in enumerate(("general", "symmetric", "skew-symmetric")):
+        n = 300 + 50 * k
+        rows = rng.integers(0, n, 3000)
+        cols = rng.integers(0, n, 3000)
+        if sym != "general":
+            rows, cols = np.maximum(rows, cols), np.minimum(rows, cols)
+            keep = rows != cols if sym == "skew-symmetric" else np.ones(rows.size, bool)
+            rows, cols = rows[keep], cols[keep]
+        lines = [f"%%MatrixMarket matrix coordinate real {sym}", f"{n} {n} {rows.size}"]
+        lines += [f"{r + 1} {c + 1} {float(v)!r}" for r, c, v in
+                  zip(rows, cols, rng.uniform(-1, 1, rows.size))]
+        p = tmp_path / f"m{k}.mtx"
+        p.write_text("\n".join(lines) + "\n")
+        paths.append(str(p))
+    bad = tmp_path / "bad.mtx"
+    bad.write_text("not a matrix\n")
+    paths.append(str(bad))
+    for op in ("non", "trans"):
+        res = mtx.run_mtx_batch(paths, value_dtype=torch.float32, warmup=1, iters=2, op=op)
+        assert [r["status"] for r in res] == ["PASS", "PASS", "PASS", "ERROR"], res
+        assert all(r["err_cu"] is not None and r["err_cu"] <= 1.0 for r in res[:3])
+    res = mtx.run_mtx_batch(paths[:1], value_dtype=torch.float64, warmup=1, iters=2)
+    assert res[0]["status"] == "PASS" and res[0]["err_pt"] <= 1.0
