@@ -84,6 +84,47 @@ def cpu_baseline(ip, cols, vals, x, nbytes, budget_s=10.0):
                        f"fp32 algorithmic bytes"}, y64)
 
 
+def measure_modes(d_vals, d_cols, d_ip, d_x, n, nnz, prepared, use_opt, steps):
+    """The same workload in the other modes, each timed like the main line:
+    the default (fp64-accumulating) path and the three split-precision value
+    storages.  Bytes per mode use that mode's stored value size."""
+    import torch
+    from flagsparse_public_amd import flagsparse_spmv_csr
+    from flagsparse_public_amd.precision import (FORMAT_BYTES, flagsparse_spmv_csr_split,
+                                                 prepare_spmv_csr_split)
+    dev = d_x.device
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(steps):
+            out = fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) * 1e3 / steps, out
+
+    out = {}
+    us, yv = timed(lambda: flagsparse_spmv_csr(x=d_x, prepared=prepared, use_opt=not use_opt))
+    b = spmv_bytes(n, nnz)
+    out["default_path" if use_opt else "opt_path"] = {
+        "us": round(us, 2), "GB/s": round(b / us / 1e3, 1), "_y": yv}
+    for fmt in ("bf16x2", "bf16", "f16_rowscale"):
+        ps = prepare_spmv_csr_split(d_vals, d_cols, d_ip, (n, n), fmt)
+        us, yv = timed(lambda: flagsparse_spmv_csr_split(d_x, ps))
+        per_entry, per_row = FORMAT_BYTES[fmt]
+        b = (n + 1) * 4 + nnz * (4 + per_entry + 4) + n * (4 + per_row)
+        out[fmt] = {"us": round(us, 2), "GB/s": round(b / us / 1e3, 1),
+                    "value_bytes_per_entry": per_entry, "_y": yv}
+        del ps
+        _log(f"mode {fmt}: {us:.1f} us/step")
+    return out
+
+
 def calibrate_copy(mib, reps):
     """Plain device-to-device copies of a known size: the reference point that
     turns rocprofv3 FETCH_SIZE / WRITE_SIZE readings into bytes."""
@@ -108,6 +149,8 @@ def main(argv=None):
     ap.add_argument("--path", choices=["opt", "default"], default="opt")
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-modes", action="store_true",
+                    help="skip the split-precision and default-path lines")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run device copies of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -236,14 +279,27 @@ def main(argv=None):
             "step_frac": round(gbs / HBM_PEAK_GBS, 4),
         },
     }
+    modes = None
+    if not args.no_modes:
+        modes = measure_modes(d_vals, d_cols, d_ip, d_x, n, nnz, prepared, use_opt,
+                              max(args.steps // 4, 10))
     if not args.no_cpu:
         cb, y64 = cpu_baseline(ip, cols, vals, x, nbytes)
         result["cpu_baseline"] = cb
-        yh = y.double().cpu().numpy()
         from oracle import oracle
         cond = oracle.spmv_abs(ip, cols, vals, x)
-        rel = np.abs(yh - y64) / np.where(cond > 0, cond, 1.0)
-        result["config"]["max_rel_err_vs_fp64"] = float(rel.max())
+
+        def rel_err(yd):
+            yh = yd.double().cpu().numpy()
+            return float((np.abs(yh - y64) / np.where(cond > 0, cond, 1.0)).max())
+        result["config"]["max_rel_err_vs_fp64"] = rel_err(y)
+        if modes is not None:
+            for m in modes.values():
+                m["max_rel_err_vs_fp64"] = rel_err(m.pop("_y"))
+    if modes is not None:
+        for m in modes.values():
+            m.pop("_y", None)
+        result["config"]["modes"] = modes
     print(json.dumps(result), flush=True)
     return 0
 

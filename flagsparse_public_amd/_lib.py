@@ -82,6 +82,7 @@ class FsPlan(ctypes.Structure):
         ("group_blocks", ctypes.c_int64),
         ("group_tiles", ctypes.c_void_p),
         ("row_sums", ctypes.c_void_p),
+        ("tile_entry", ctypes.c_void_p),
     ]
 
 

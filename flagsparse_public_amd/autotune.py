@@ -23,7 +23,7 @@ import torch
 
 from .rowsplit import Schedule
 
-VARIANTS_VERSION = 2
+VARIANTS_VERSION = 3
 
 # SpMV (fp32, native accumulation) variants.  0-5: entry tiles (fixed runs of
 # entries), 3-5 with the columns cut into L2-sized slices when x is larger
@@ -47,16 +47,26 @@ SPMV_VARIANTS = (
     Schedule(group_lanes=32, short_max=512, chunk_nnz=1024, unroll=4),
     Schedule(group_lanes=64, short_max=4096, chunk_nnz=4096, unroll=8),
 )
-# The plain default-tile SpMV the learned selector is compared with: one
-# 8-lane group per row with fixed bounds (the analogue of the reference's
-# default-path kernel with its fixed 256-entry tile, spmv_csr.py:158-184).
-SPMV_DEFAULT_INDEX = 12
+# The plain default-tile SpMV the learned selector is compared with: a whole
+# wavefront per row over fixed blocks of entries -- the structure of the
+# reference's default path, one program per row stepping through 256-entry
+# blocks (spmv_csr.py:158-184, 664-708).
+SPMV_DEFAULT_INDEX = 15
+# A tuned fixed schedule reported beside it: 8-lane groups per row.
+SPMV_FIXED_S8_INDEX = 12
 
 
-def spmv_heuristic_index(n_cols: int) -> int:
-    """Variant equal to the fixed rule of rowsplit.default_spmv_schedule for
-    fp32 (entry tiles of 2048; 2 MB column slices once x exceeds 4 MB)."""
-    return 3 if n_cols * 4 > (4 << 20) else 1
+def spmv_heuristic_index(n_cols: int, n_rows: int | None = None, nnz: int | None = None) -> int:
+    """Variant chosen by the fixed rule of rowsplit.default_spmv_schedule for
+    fp32: entry tiles over 2 MB column slices once x exceeds 8 MB, else row
+    groups of width pow2ceil(mean row length / 2) (the nearest variant)."""
+    if n_cols * 4 > (8 << 20):
+        return 3
+    if n_rows is None or nnz is None:
+        return SPMV_FIXED_S8_INDEX
+    from .rowsplit import rowgroup_spmv_schedule
+    s = rowgroup_spmv_schedule(n_rows, nnz).group_lanes
+    return {1: 9, 2: 10, 4: 11, 8: 12, 16: 13, 32: 14, 64: 15}[s]
 
 
 # SpMM (64 dense columns) variants: column tile (lane-group width), short-row

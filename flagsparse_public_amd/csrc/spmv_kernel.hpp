@@ -81,6 +81,7 @@ struct SpmvParams {
   int64_t tile_nnz;            // tile role: target entries per tile (0: row-group role)
   int64_t n_tiles;
   const int64_t* tile_start;   // [n_tiles + 1] first row of each tile
+  const int64_t* tile_entry;   // entry tiles: [n_tiles + 1] first entry of each tile
   int64_t nnz;
   int n_groups;                // entry tiles: block groups (0: block b = tile b)
   const int64_t* group_tiles;  // [n_groups + 1]
@@ -335,6 +336,7 @@ int64_t spmv_fill_schedule(SpmvParams<VS, V, I, P>& p, const fs_csr* M, const fs
   p.tile_nnz = pl->tile_nnz;
   p.n_tiles = pl->n_tiles;
   p.tile_start = pl->tile_start;
+  p.tile_entry = pl->tile_entry;
   p.nnz = M->nnz;
   p.n_groups = pl->n_groups;
   p.group_tiles = pl->group_tiles;
@@ -357,12 +359,15 @@ void spmv_launch_tiles(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream
 
 // ------------------------------------------------------------ entry tiles
 //
-// Tile t holds the entries [t*T, min((t+1)*T, nnz)) (T = tile_nnz = E * kBlock)
-// and owns the rows r with t*T <= row_ptr[r] < (t+1)*T (the last tile also
-// owns the rows that start at nnz); tile_start[t] is its first owned row.  The
-// entry range of a tile is known at launch, so every thread issues all E of
-// its col / value loads (16-byte vector loads when VEC) before the first x
-// gather, and the first batch of row pointers is loaded beside them.  Products
+// Tile t holds the entries [tile_entry[t], tile_entry[t+1]) -- at most T =
+// tile_nnz = E * kBlock of them -- and owns the rows tile_start[t] ..
+// tile_start[t+1] - 1, which start inside its entry range (the last tile also
+// owns the rows that start at nnz).  The plan cuts the entries every T and
+// cuts again every kSegRows owned rows, so a run of empty rows is spread over
+// many blocks.  The entry range is known from two scalar loads, so every
+// thread issues all E of its col / value loads (16-byte vector loads when VEC
+// and the tile is full and aligned) before the first x gather, and the first
+// batch of row pointers is loaded beside them.  Products
 // go to LDS in entry order; every owned row is summed there in fp64 (complex:
 // two fp64) in entry order -- one thread per row up to kThreadRowMax entries,
 // one wavefront above.  Two pieces can run past a tile: the head (entries of
@@ -417,9 +422,13 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I,
     t = p.group_tiles[g] + t / p.n_groups;
     if (t >= p.group_tiles[g + 1] || t >= p.n_tiles) return;
   }
-  const int64_t e0 = t * T;
-  const int64_t rem = p.nnz - e0;
-  const int len = rem < T ? (int)(rem > 0 ? rem : 0) : T;
+  // entry range [e0, e0 + len): len <= T (clamped, so a damaged plan cannot
+  // overrun the LDS products)
+  const int64_t e0 = p.tile_entry[t];
+  int64_t e1 = p.tile_entry[t + 1];
+  e1 = e1 > p.nnz ? p.nnz : e1;
+  const int64_t span = e1 - e0;
+  const int len = span <= 0 ? 0 : (span < T ? (int)span : T);
   // owned rows [r0, r1), clamped to [0, n_rows] so a damaged plan cannot
   // send a row-pointer read out of bounds
   int64_t r0 = p.tile_start[t], r1 = p.tile_start[t + 1];
@@ -433,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I,
     const int i = k * kBlock + tid;
     pv[k] = i <= nb0 ? (int64_t)p.ptr[r0 + i] : 0;
   }
-  if (len == T) {
+  if (len == T && (!VEC || (e0 & 3) == 0)) {
     I col[E];
     C a[E];
     if constexpr (VEC) {
@@ -557,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS,
   if (t + 1 >= p.n_tiles) return;  // the last tile ends at nnz: no row runs past it
   const int64_t r0 = p.tile_start[t], r1 = p.tile_start[t + 1];
   if (r1 <= r0 || r1 > p.n_rows) return;
-  if ((int64_t)p.ptr[r1] <= (t + 1) * p.tile_nnz) return;
+  if ((int64_t)p.ptr[r1] <= p.tile_entry[t + 1]) return;
   const S* carries = reinterpret_cast<const S*>(p.partials);
   S acc = carries[2 * t + 1];
   for (int64_t u = t + 1; u < p.n_tiles; ++u) {

@@ -72,11 +72,12 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
   if (P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES) {
     if (n_dense != 1) return fail(FS_ERR_VALUE, "%s: entry tiles are for SpMV", who);
     if (P->n_long != 0) return fail(FS_ERR_VALUE, "%s: entry tiles take no long-row lists", who);
-    const int64_t want = A->nnz > 0 ? (A->nnz + P->tile_nnz - 1) / P->tile_nnz : 1;
-    if (P->n_tiles != want)
-      return fail(FS_ERR_VALUE, "%s: entry tiles need n_tiles = %lld, got %lld", who,
-                  (long long)want, (long long)P->n_tiles);
-    if (!P->tile_start) return fail(FS_ERR_VALUE, "%s: entry tiles need tile_start", who);
+    const int64_t least = A->nnz > 0 ? (A->nnz + P->tile_nnz - 1) / P->tile_nnz : 1;
+    if (P->n_tiles < least)
+      return fail(FS_ERR_VALUE, "%s: entry tiles need n_tiles >= %lld, got %lld", who,
+                  (long long)least, (long long)P->n_tiles);
+    if (!P->tile_start || !P->tile_entry)
+      return fail(FS_ERR_VALUE, "%s: entry tiles need tile_start and tile_entry", who);
     if (P->n_groups < 0 || (P->n_groups > 0 && (!P->group_tiles || P->group_blocks < 0)))
       return fail(FS_ERR_VALUE, "%s: bad block groups", who);
     if (A->value_type != FS_F32 && A->value_type != FS_F64 && A->value_type != FS_C64 &&

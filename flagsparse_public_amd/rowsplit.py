@@ -50,21 +50,22 @@ def _pow2_ceil(v: float) -> int:
 TILE_ROWS = 512  # rows per tile at most (the kernel's LDS row-pointer slots)
 
 
-L2_SLICE_BYTES = 2 << 20   # x columns per XCD slice: half of one XCD's 4 MB L2
-SLICE_MIN_X_BYTES = 4 << 20  # slice only when x is larger than one L2
+L2_SLICE_BYTES = 2 << 20    # x columns per XCD slice: half of one XCD's 4 MB L2
+SLICE_MIN_X_BYTES = 8 << 20  # slice only when x is well beyond what L2 + MALL serve well
 
 
 def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None, n_cols: int | None = None
                           ) -> Schedule:
-    """Fixed rule used when no selector is supplied: entry tiles of 2048
-    entries; when x is larger than one XCD's L2 (fp32 / fp64 / complex values),
-    the native path also cuts the columns into slices of 2 MB of x."""
-    slice_cols = 0
+    """Fixed rule used when no selector is supplied.  x larger than 8 MB
+    (fp32 / fp64 / complex values): entry tiles of 2048 over the column-sliced
+    layout (2 MB of x per slice) -- random x gathers are then served from each
+    XCD's L2.  Otherwise one lane group per row, its width from the mean row
+    length (rowgroup_spmv_schedule)."""
     if value_dtype is not None and n_cols is not None:
         item = torch.empty(0, dtype=value_dtype).element_size()
         if n_cols * item > SLICE_MIN_X_BYTES:
-            slice_cols = L2_SLICE_BYTES // item
-    return Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=slice_cols)
+            return Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=L2_SLICE_BYTES // item)
+    return rowgroup_spmv_schedule(n_rows, nnz)
 
 
 def row_tile_spmv_schedule(n_rows: int, nnz: int, value_dtype=None) -> Schedule:
@@ -163,21 +164,45 @@ class RowSplitPlan:
             self.tile_start = None
             self.n_tiles = 0
 
+    ENTRY_ROW_CAP = 1024  # owned rows per entry tile (the kernel's LDS row batch)
+
     def _init_entry_tiles(self, indptr, n_rows, schedule):
-        """Entry tiles: tile t holds entries [t*T, (t+1)*T) and owns the rows
-        whose first entry falls in it (the last tile also the rows that start
-        at nnz)."""
+        """Entry tiles: cut the entries every T and, inside a tile that owns
+        more than ENTRY_ROW_CAP rows (runs of empty or one-entry rows), cut
+        again every ENTRY_ROW_CAP rows at a row start.  Tile t holds entries
+        [tile_entry[t], tile_entry[t+1]) and owns rows [tile_start[t],
+        tile_start[t+1]), which start in that range (the last tile also owns
+        the rows that start at nnz)."""
         device = indptr.device
         T = int(schedule.tile_nnz)
-        nnz = int(indptr[-1].item()) if n_rows > 0 else 0
-        self.n_tiles = max(1, -(-nnz // T))
-        starts = torch.zeros(self.n_tiles + 1, dtype=torch.int64, device=device)
-        starts[-1] = n_rows
-        if self.n_tiles > 1:
-            bounds = torch.arange(1, self.n_tiles, dtype=torch.int64, device=device) * T
-            starts[1:-1] = torch.searchsorted(indptr[:-1].to(torch.int64).contiguous(), bounds,
-                                              side="left")
-        self.tile_start = starts
+        cap = self.ENTRY_ROW_CAP
+        ptr = indptr.to(torch.int64)
+        nnz = int(ptr[-1].item()) if n_rows > 0 else 0
+        nb = max(1, -(-nnz // T))
+        R = torch.zeros(nb + 1, dtype=torch.int64, device=device)
+        R[-1] = n_rows
+        if nb > 1:
+            bounds = torch.arange(1, nb, dtype=torch.int64, device=device) * T
+            R[1:-1] = torch.searchsorted(ptr[:-1].contiguous(), bounds, side="left")
+        E = torch.arange(nb + 1, dtype=torch.int64, device=device) * T
+        E[-1] = nnz
+        counts = R[1:] - R[:-1]
+        extra = torch.clamp((counts + cap - 1) // cap - 1, min=0)
+        n_extra = int(extra.sum().item())
+        if n_extra:
+            base = torch.repeat_interleave(torch.arange(nb, device=device), extra)
+            first = torch.cumsum(extra, 0) - extra          # first extra of each base tile
+            k = torch.arange(n_extra, device=device) - first[base] + 1
+            R_x = R[base] + k * cap
+            E_x = ptr[R_x]
+            # boundaries are monotone in both coordinates: order by (entry, row)
+            key = torch.cat([E[:-1], E_x]) * (n_rows + 1) + torch.cat([R[:-1], R_x])
+            order = torch.argsort(key, stable=True)
+            R = torch.cat([torch.cat([R[:-1], R_x])[order], R[-1:]])
+            E = torch.cat([torch.cat([E[:-1], E_x])[order], E[-1:]])
+        self.n_tiles = int(R.numel()) - 1
+        self.tile_start = R.contiguous()
+        self.tile_entry = E.contiguous()
         self.long_rows = torch.zeros(0, dtype=torch.int64, device=device)
         self.long_chunk_ptr = torch.zeros(1, dtype=torch.int64, device=device)
         self.chunk_long = torch.zeros(0, dtype=torch.int32, device=device)
@@ -222,7 +247,8 @@ class RowSplitPlan:
                              tile_mode=_lib.FS_TILE_ENTRIES, n_groups=self.n_groups,
                              group_blocks=self.group_blocks,
                              group_tiles=_lib.ptr(self.group_tiles),
-                             row_sums=_lib.ptr(self.row_sums))
+                             row_sums=_lib.ptr(self.row_sums),
+                             tile_entry=_lib.ptr(self.tile_entry))
             self._structs[key] = st
             return st
         if self.n_long:

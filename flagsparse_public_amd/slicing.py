@@ -96,7 +96,9 @@ class SlicedCsr:
         self.schedule = Schedule(tile_nnz=T, tile_mode=1)
         self.plan = RowSplitPlan(self.cptr, self.n_compact, value_dtype, self.schedule, accumulate)
         qn = P // N_GROUPS
-        first = slice_entry[torch.arange(0, P, qn, device=dev)] // T
+        # group g starts at the tile holding the first entry of slice g*q
+        starts = slice_entry[torch.arange(0, P, qn, device=dev)].contiguous()
+        first = torch.searchsorted(self.plan.tile_entry[1:].contiguous(), starts, side="right")
         gt = torch.empty(N_GROUPS + 1, dtype=torch.int64, device=dev)
         gt[:N_GROUPS] = torch.clamp(first, max=self.plan.n_tiles)
         gt[N_GROUPS] = self.plan.n_tiles

@@ -27,8 +27,9 @@ import numpy as np
 import torch
 
 from . import _lib, synthetic
-from .autotune import (SPMM_DEFAULT_INDEX, SPMM_VARIANTS, SPMV_DEFAULT_INDEX, SPMV_VARIANTS,
-                       TimingDB, spmv_heuristic_index, tune_spmm, tune_spmv)
+from .autotune import (SPMM_DEFAULT_INDEX, SPMM_VARIANTS, SPMV_DEFAULT_INDEX,
+                       SPMV_FIXED_S8_INDEX, SPMV_VARIANTS, TimingDB, spmv_heuristic_index,
+                       tune_spmm, tune_spmv)
 from .selector import Selector
 from .sparse_operations.spmm_csr import prepare_spmm_csr_opt
 from .sparse_operations.spmv_csr import prepare_spmv_csr
@@ -105,10 +106,13 @@ def evaluate(sel: Selector, db: TimingDB, default_index: int, device, heuristic=
         "per_family": {},
     }
     if heuristic is not None:
-        hidx = np.asarray([heuristic(int(n)) for n in sizes[:, 0]])
+        hidx = np.asarray([heuristic(int(n), int(n), int(z)) for n, z in sizes])
         t_heu = times[rows, hidx]
         out["geomean_speedup_vs_heuristic"] = float(np.exp(np.mean(np.log(t_heu / t_sel))))
         out["heuristic_geomean_regret"] = float(np.exp(np.mean(np.log(t_heu / t_best))))
+        t_s8 = times[:, SPMV_FIXED_S8_INDEX]
+        out["geomean_speedup_vs_fixed_s8"] = float(np.exp(np.mean(np.log(t_s8 / t_sel))))
+        out["fixed_s8_geomean_regret"] = float(np.exp(np.mean(np.log(t_s8 / t_best))))
     for fam in sorted(set(fams.tolist())):
         m = fams == fam
         out["per_family"][fam] = {

@@ -123,6 +123,7 @@ typedef struct fs_plan {
   int64_t group_blocks;          /* blocks per group (largest tile range)  */
   const int64_t *group_tiles;    /* [n_groups + 1] tile range of a group   */
   void *row_sums;                /* FS_TILE_ENTRIES: if set, raw row sums  */
+  const int64_t *tile_entry;     /* FS_TILE_ENTRIES: [n_tiles + 1]         */
 } fs_plan;
 
 /*
@@ -130,13 +131,17 @@ typedef struct fs_plan {
  *   FS_TILE_ROWS     tiles of consecutive short rows holding about tile_nnz
  *                    entries; rows over short_max go to the listed long-row
  *                    chunks.
- *   FS_TILE_ENTRIES  tile t holds exactly the entries [t*tile_nnz,
- *                    (t+1)*tile_nnz) and owns the rows that start in it;
- *                    n_tiles = max(1, ceil(nnz / tile_nnz)), tile_start[t] =
- *                    first owned row (tile_start[n_tiles] = n_rows), no long-
- *                    row lists (n_long = 0), partials = two carries per tile
- *                    (fs_tile_carry_bytes).  tile_nnz is 1024, 2048 or 4096
- *                    for fp32 / fp64 values with int32 columns, 2048 otherwise.
+ *   FS_TILE_ENTRIES  tile t holds the entries [tile_entry[t],
+ *                    tile_entry[t+1]) -- at most tile_nnz -- and owns the rows
+ *                    tile_start[t] .. tile_start[t+1]-1, which start inside
+ *                    that range (the last tile also owns rows starting at
+ *                    nnz); tile_entry[0] = 0, tile_entry[n_tiles] = nnz,
+ *                    tile_start[n_tiles] = n_rows, both nondecreasing.  The
+ *                    planner cuts every tile_nnz entries and every 1024 owned
+ *                    rows.  No long-row lists (n_long = 0); partials = two
+ *                    carries per tile (fs_tile_carry_bytes).  tile_nnz is
+ *                    1024, 2048 or 4096 for fp32 / fp64 values with int32
+ *                    columns, 2048 otherwise.
  *                    With n_groups = G > 0, the grid is G * group_blocks
  *                    blocks; block b serves group g = b % G and takes tile
  *                    group_tiles[g] + b / G (blocks past group_tiles[g + 1]

@@ -219,12 +219,12 @@ def test_default_schedules():
     assert rowsplit.rowgroup_spmv_schedule(10, 10).group_lanes == 1
     assert rowsplit.rowgroup_spmv_schedule(10, 100000).group_lanes == 64
     d = rowsplit.default_spmv_schedule(4_000_000, 48_000_000)
-    assert d.tile_nnz == 2048 and d.tile_mode == 1 and d.x_slice_cols == 0
-    # x of 16 MB (fp32): slices of 2 MB of x; 2 MB x: no slicing
+    assert d == rowsplit.rowgroup_spmv_schedule(4_000_000, 48_000_000)
+    # x of 16 MB (fp32): entry tiles over slices of 2 MB of x; 4 MB x: row groups
     d = rowsplit.default_spmv_schedule(4_000_000, 48_000_000, torch.float32, 4_000_000)
-    assert d.x_slice_cols == 512 * 1024
-    assert rowsplit.default_spmv_schedule(500_000, 10, torch.float32, 500_000).x_slice_cols == 0
-    assert rowsplit.default_spmv_schedule(10, 10, torch.float64, 600_000).x_slice_cols == 256 * 1024
+    assert d.x_slice_cols == 512 * 1024 and d.tile_mode == 1 and d.tile_nnz == 2048
+    assert rowsplit.default_spmv_schedule(10**6, 10, torch.float32, 10**6).x_slice_cols == 0
+    assert rowsplit.default_spmv_schedule(10, 10, torch.float64, 1_100_000).x_slice_cols == 256 * 1024
     r = rowsplit.row_tile_spmv_schedule(10, 10, torch.complex64)
     assert r.tile_nnz == 1024 and r.short_max <= r.tile_nnz and r.chunk_nnz % 64 == 0
 
@@ -251,25 +251,38 @@ def test_tile_starts_invariants(seed):
 @pytest.mark.parametrize("seed", [0, 1, 2])
 @pytest.mark.parametrize("T", [1024, 2048, 4096])
 def test_entry_tile_ownership(seed, T):
-    """Entry tiles: tile t owns exactly the rows whose first entry lies in
-    [t*T, (t+1)*T); the last tile also owns the rows that start at nnz; every
-    row is owned once (brute force over the rows)."""
+    """Entry tiles, brute force over rows and entries: the entry ranges
+    partition [0, nnz) in pieces of at most T, the owned-row ranges partition
+    the rows in pieces of at most 1024, every owned row starts inside its
+    tile's entry range (an empty row may sit on the end), every owned row
+    but the last ends inside it, and the cuts fall every T entries plus at
+    row starts inside row-heavy tiles."""
     rng = np.random.default_rng(seed)
     lens = rng.poisson(5, 6000)
     lens[rng.random(6000) < 0.3] = 0
     lens[rng.integers(0, 6000, 4)] = [9000, 4096, 2 * T, 1]
-    lens[-50:] = 0  # trailing empty rows start at nnz
+    lens[1000:4000:2] = 0  # a long stretch of mostly empty rows
+    lens = np.concatenate([lens, np.zeros(20000, np.int64)])  # trailing empty rows
     ip = np.zeros(lens.size + 1, np.int64)
     np.cumsum(lens, out=ip[1:])
     nnz = int(ip[-1])
+    n = lens.size
     sch = rowsplit.Schedule(tile_nnz=T, tile_mode=1)
-    plan = rowsplit.RowSplitPlan(torch.from_numpy(ip), lens.size, torch.float32, sch, 0)
-    ts = plan.tile_start.tolist()
-    assert plan.n_tiles == max(1, -(-nnz // T)) and len(ts) == plan.n_tiles + 1
-    assert ts[0] == 0 and ts[-1] == lens.size and ts == sorted(ts)
-    owner = np.minimum(ip[:-1] // T, plan.n_tiles - 1)
+    plan = rowsplit.RowSplitPlan(torch.from_numpy(ip), n, torch.float32, sch, 0)
+    R = plan.tile_start.numpy()
+    E = plan.tile_entry.numpy()
+    assert R[0] == 0 and R[-1] == n and E[0] == 0 and E[-1] == nnz
+    assert np.all(np.diff(R) >= 0) and np.all(np.diff(E) >= 0)
+    assert np.all(np.diff(E) <= T) and np.all(np.diff(R) <= rowsplit.RowSplitPlan.ENTRY_ROW_CAP)
+    assert set((np.arange(1, -(-nnz // T)) * T).tolist()) <= set(E.tolist())
     for t in range(plan.n_tiles):
-        assert np.all(owner[ts[t]:ts[t + 1]] == t)
+        rows = np.arange(R[t], R[t + 1])
+        if rows.size == 0:
+            continue
+        assert np.all(ip[rows] >= E[t]) and np.all(ip[rows] <= E[t + 1])
+        nonempty = lens[rows] > 0
+        assert np.all(ip[rows][nonempty] < E[t + 1])
+        assert np.all(ip[rows[:-1] + 1] <= E[t + 1])
     assert plan.n_long == 0
 
 
@@ -277,6 +290,7 @@ def test_entry_tile_empty_matrix():
     ip = torch.zeros(7, dtype=torch.int64)
     plan = rowsplit.RowSplitPlan(ip, 6, torch.float64, rowsplit.Schedule(tile_nnz=2048, tile_mode=1), 0)
     assert plan.n_tiles == 1 and plan.tile_start.tolist() == [0, 6]
+    assert plan.tile_entry.tolist() == [0, 0]
 
 
 def test_sliced_layout_structure():

@@ -1,0 +1,81 @@
+"""What the trained selector gains, checked on the CPU from committed data.
+
+The timing databases (flagsparse_public_amd/selector_models/*_db.npz) hold the
+autotuner's per-variant device times for held-out matrices (seeds 7000..) and
+the training matrices (5000..); the models (*_f32_mlp.json) were trained on the
+training rows by the library's HIP MLP kernels.  Here the oracle's MLP forward
+(oracle/csr_oracle.c) recomputes every held-out choice from the stored
+features, and the test checks
+  * the selection matches the one recorded on the device at evaluation time
+    (same regret / speed-up as eval.json, to rounding);
+  * regret against exhaustive autotuning (time of the chosen variant over the
+    fastest) and the speed-up over the plain default-tile variant and over
+    the fixed heuristic rule -- the gain of the network over the methods
+    without it.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from flagsparse_public_amd.autotune import (SPMM_DEFAULT_INDEX, SPMV_DEFAULT_INDEX, TimingDB,
+                                            default_db_dir, spmv_heuristic_index)
+
+DIR = default_db_dir()
+
+
+def _have(*names):
+    return all(os.path.exists(os.path.join(DIR, n)) for n in names)
+
+
+def _unpack(params, i, h, o):
+    p = np.asarray(params, np.float64)
+    sizes = [(i, h), (h,), (h, h), (h,), (h, o), (o,)]
+    out, k = [], 0
+    for s in sizes:
+        n = int(np.prod(s))
+        out.append(p[k:k + n].reshape(s))
+        k += n
+    return out
+
+
+def _choices(orc, model, feats):
+    mu = np.asarray(model["mu"], np.float64)
+    sd = np.asarray(model["sd"], np.float64)
+    # the device standardises fp32 features with fp32 mu / sd
+    Z = ((feats.astype(np.float32) - mu.astype(np.float32)) / sd.astype(np.float32))
+    W = _unpack(model["params"], model["in_dim"], model["hidden"], model["out_dim"])
+    _, _, Y = orc.mlp_forward(Z.astype(np.float64), *W)
+    return np.argmin(Y, axis=1), Y
+
+
+@pytest.mark.parametrize("op,default_index", [("spmv", SPMV_DEFAULT_INDEX),
+                                              ("spmm64", SPMM_DEFAULT_INDEX)])
+def test_selector_gain_on_held_out(orc, op, default_index):
+    if not _have(f"{op}_test_db.npz", f"{op}_f32_mlp.json", "eval.json"):
+        pytest.skip("no trained selector committed for " + op)
+    db = TimingDB.load(os.path.join(DIR, f"{op}_test_db.npz"))
+    with open(os.path.join(DIR, f"{op}_f32_mlp.json")) as fh:
+        model = json.load(fh)
+    with open(os.path.join(DIR, "eval.json")) as fh:
+        ev = json.load(fh)[op]["held_out"]
+    _, _, feats, times, sizes = db.arrays()
+    choice, Y = _choices(orc, model, feats)
+    rows = np.arange(times.shape[0])
+    t_sel = times[rows, choice]
+    t_best = times.min(axis=1)
+    regret = float(np.exp(np.mean(np.log(t_sel / t_best))))
+    speedup = float(np.exp(np.mean(np.log(times[:, default_index] / t_sel))))
+    # the CPU forward reproduces the device's choices (ties within fp32 rounding aside)
+    assert abs(regret - ev["geomean_regret"]) <= 2e-3 * ev["geomean_regret"]
+    assert abs(speedup - ev["geomean_speedup_vs_default"]) <= 2e-3 * speedup
+    # the network gains over the plain default-tile variant and stays close to
+    # exhaustive autotuning
+    assert speedup > 1.0
+    assert regret < float(np.exp(np.mean(np.log(times[:, default_index] / t_best))))
+    if op == "spmv":
+        heur = np.asarray([spmv_heuristic_index(int(n), int(n), int(z)) for n, z in sizes])
+        t_heu = times[rows, heur]
+        assert float(np.exp(np.mean(np.log(t_heu / t_sel)))) >= 0.97
