@@ -8,6 +8,7 @@ does not load the HIP library until an operator is used.
 __version__ = "0.1.0"
 
 _OPS_EXPORTS = {
+    "PreparedCoo",
     "PreparedCsrSpmv",
     "PreparedCsrSpmmOpt",
     "PreparedCsrSpmmOptAlg2",
@@ -21,10 +22,12 @@ _OPS_EXPORTS = {
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
     "flagsparse_spmm_csr_opt_alg2",
+    "flagsparse_spmv_coo",
     "flagsparse_spmv_coo_tocsr",
     "flagsparse_spmv_csr",
     "prepare_spmm_csr_opt",
     "prepare_spmm_csr_opt_alg2",
+    "prepare_spmv_coo",
     "prepare_spmv_coo_tocsr",
     "prepare_spmv_csr",
 }

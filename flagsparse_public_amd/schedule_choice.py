@@ -5,8 +5,9 @@
   * "default" is the fixed rule of ``rowsplit``;
   * "learned" (also what None means) runs the packaged selector for ``op`` on
     the device -- features from the analysis pass's statistics, MLP forward,
-    argmin -- and falls back to the fixed rule when no model is installed or
-    the values are not fp32;
+    argmin -- and falls back to the fixed rule when no model is installed, the
+    values are not fp32, or the matrix is larger than any the model was
+    trained on;
   * "exhaustive" is resolved after the prepared object exists
     (``exhaustive_spmv_schedule`` times every variant).
 Returns (schedule or None, how).
@@ -42,6 +43,9 @@ def choose_schedule(op, request, data, n_rows, n_cols, stats_dev):
     sel = load_default(op, data.device)
     if sel is None:
         return fallback, "default"
+    if not sel.covers(n_rows, n_cols, nnz):
+        # outside the sizes the network was trained on: the fixed rule
+        return fallback, "default (outside the selector's training range)"
     sch, idx = sel.schedule_for(stats_dev, n_rows, n_cols, nnz)
     return sch, f"learned:{idx}"
 

@@ -151,6 +151,17 @@ class Selector:
         _, _, Y = self.mlp.forward(torch.from_numpy(Z.astype(np.float32)).to(self.mlp.device))
         return Y.double().cpu().numpy()
 
+    def covers(self, n_rows: int, n_cols: int, nnz: int) -> bool:
+        """Whether the matrix lies inside the training set's size range
+        (meta "n_max" / "nnz_max"; a model without them covers everything)."""
+        n_max = self.meta.get("n_max")
+        nnz_max = self.meta.get("nnz_max")
+        if n_max is not None and max(n_rows, n_cols) > 1.1 * float(n_max):
+            return False
+        if nnz_max is not None and nnz > 1.1 * float(nnz_max):
+            return False
+        return True
+
     def select_on_device(self, feats_dev):
         """Variant index (device int32 tensor) for one feature vector on the device."""
         choice = torch.empty(1, dtype=torch.int32, device=self.mlp.device)

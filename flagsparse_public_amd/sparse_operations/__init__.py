@@ -22,8 +22,10 @@ from .spmm_csr_opt_alg2 import (
     prepare_spmm_csr_opt_alg2,
 )
 from .benchmarks import benchmark_spmm_case, benchmark_spmv_case, comprehensive_spmm_test
+from .spmv_coo import PreparedCoo, flagsparse_spmv_coo, prepare_spmv_coo
 
 __all__ = [
+    "PreparedCoo",
     "PreparedCsrSpmv",
     "PreparedCsrSpmmOpt",
     "PreparedCsrSpmmOptAlg2",
@@ -37,10 +39,12 @@ __all__ = [
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
     "flagsparse_spmm_csr_opt_alg2",
+    "flagsparse_spmv_coo",
     "flagsparse_spmv_coo_tocsr",
     "flagsparse_spmv_csr",
     "prepare_spmm_csr_opt",
     "prepare_spmm_csr_opt_alg2",
+    "prepare_spmv_coo",
     "prepare_spmv_coo_tocsr",
     "prepare_spmv_csr",
 ]

@@ -171,7 +171,9 @@ def main(argv=None):
         _, _, Xtr, Ttr, _ = dbs["train"][op].arrays()
         sel = Selector.train(op, Xtr, Ttr, device, epochs=args.epochs,
                              log=lambda m, op=op: _log(f"{op}: {m}"))
-        sel.meta.update({"device": name, "default_index": default_index})
+        _, _, _, _, sizes = dbs["train"][op].arrays()
+        sel.meta.update({"device": name, "default_index": default_index,
+                         "n_max": int(sizes[:, 0].max()), "nnz_max": int(sizes[:, 1].max())})
         heur = spmv_heuristic_index if op == "spmv" else None
         ev, _ = evaluate(sel, dbs["test"][op], default_index, device, heur)
         tr, _ = evaluate(sel, dbs["train"][op], default_index, device, heur)
