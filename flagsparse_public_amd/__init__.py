@@ -19,6 +19,7 @@ _OPS_EXPORTS = {
     "benchmark_spmm_opt_case",
     "benchmark_spmv_case",
     "comprehensive_spmm_test",
+    "flagsparse_spmm_coo",
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
     "flagsparse_spmm_csr_opt_alg2",

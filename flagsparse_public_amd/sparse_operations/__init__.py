@@ -23,6 +23,7 @@ from .spmm_csr_opt_alg2 import (
 )
 from .benchmarks import benchmark_spmm_case, benchmark_spmv_case, comprehensive_spmm_test
 from .spmv_coo import PreparedCoo, flagsparse_spmv_coo, prepare_spmv_coo
+from .spmm_coo import flagsparse_spmm_coo
 
 __all__ = [
     "PreparedCoo",
@@ -36,6 +37,7 @@ __all__ = [
     "benchmark_spmm_opt_case",
     "benchmark_spmv_case",
     "comprehensive_spmm_test",
+    "flagsparse_spmm_coo",
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
     "flagsparse_spmm_csr_opt_alg2",

@@ -97,3 +97,52 @@ def test_coo_spmv_validation():
         flagsparse_spmv_coo(x=x, prepared=p, block_inner=3)
     with pytest.raises(ValueError, match="does not match prepared.op"):
         flagsparse_spmv_coo(x=x, prepared=p, op="trans")
+
+
+@pytest.mark.parametrize("dtype,npd,tol", [(torch.float32, np.float32, 1e-6),
+                                           (torch.float64, np.float64, 1e-12),
+                                           (torch.complex64, np.complex64, 1e-6),
+                                           (torch.float16, np.float16, 2e-3)],
+                         ids=["f32", "f64", "c64", "f16"])
+@pytest.mark.parametrize("idx", [torch.int32, torch.int64], ids=["i32", "i64"])
+def test_coo_spmm_matches_oracle(dtype, npd, tol, idx):
+    """flagsparse_spmm_coo (spmm_coo.py:671-695): widened accumulation, repeated
+    pairs summed, against the fp64 product."""
+    from flagsparse_public_amd import flagsparse_spmm_coo
+    rng = np.random.default_rng(11)
+    n, m, N = 400, 300, 24
+    r, c, v = _coo(rng, n, m, 3000, npd)
+    Bh = rng.uniform(-1, 1, (m, N))
+    if dtype.is_complex:
+        Bh = Bh + 1j * rng.uniform(-1, 1, (m, N))
+    Bh = Bh.astype(npd)
+    wide = np.complex128 if dtype.is_complex else np.float64
+    A = sp.coo_matrix((v.astype(wide), (r, c)), shape=(n, m)).tocsr()
+    C64 = A @ Bh.astype(wide)
+    S = abs(A) @ np.abs(Bh.astype(wide))
+    C = flagsparse_spmm_coo(torch.from_numpy(v).to(DEV), torch.from_numpy(r).to(idx).to(DEV),
+                            torch.from_numpy(c).to(idx).to(DEV), torch.from_numpy(Bh).to(DEV),
+                            (n, m))
+    assert C.dtype == dtype and tuple(C.shape) == (n, N)
+    got = C.cpu().numpy().astype(wide)
+    assert (np.abs(got - C64) / np.maximum(S, 1e-300)).max() <= tol
+    out = torch.empty((n, N), dtype=dtype, device=DEV)
+    res, ms = flagsparse_spmm_coo(torch.from_numpy(v).to(DEV), torch.from_numpy(r).to(idx).to(DEV),
+                                  torch.from_numpy(c).to(idx).to(DEV),
+                                  torch.from_numpy(Bh).to(DEV), (n, m), out=out, return_time=True)
+    assert res is out and ms >= 0 and torch.equal(out, C)
+
+
+def test_coo_spmm_validation():
+    from flagsparse_public_amd import flagsparse_spmm_coo
+    v = torch.tensor([1.0], device=DEV)
+    i = torch.tensor([0], dtype=torch.int32, device=DEV)
+    B = torch.ones((2, 3), device=DEV)
+    with pytest.raises(ValueError, match="B.shape\\[0\\] must be n_cols=3"):
+        flagsparse_spmm_coo(v, i, i, B, (2, 3))
+    with pytest.raises(TypeError, match="B dtype must match data dtype"):
+        flagsparse_spmm_coo(v, i, i, B.double(), (2, 2))
+    with pytest.raises(IndexError, match="row indices out of range for n_rows"):
+        flagsparse_spmm_coo(v, i + 4, i, B, (2, 2))
+    with pytest.raises(ValueError, match="block_n must be positive"):
+        flagsparse_spmm_coo(v, i, i, B, (2, 2), block_n=0)
