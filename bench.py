@@ -125,6 +125,125 @@ def measure_modes(d_vals, d_cols, d_ip, d_x, n, nnz, prepared, use_opt, steps):
     return out
 
 
+def _event_time(fn, steps, stream):
+    import torch
+    for _ in range(3):
+        fn()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(steps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) * 1e3 / steps
+
+
+def vendor_spmv(d_vals, d_cols, d_ip, d_x, n, nbytes, steps):
+    """rocSPARSE CSR SpMV (adaptive algorithm, analysis done once) on the
+    same matrix and x: the GPU vendor baseline."""
+    import torch
+    from flagsparse_public_amd import vendor
+    if not vendor.available():
+        return {"library": "rocSPARSE", "unavailable": vendor.unavailable_reason()}
+    y = torch.empty(n, device=d_x.device)
+    v = vendor.VendorSpmv(d_vals, d_cols, d_ip, (n, n), d_x, y, "adaptive")
+    us = _event_time(v.run, steps, torch.cuda.current_stream(d_x.device))
+    _log(f"rocSPARSE adaptive: {us:.1f} us/step")
+    return {"library": "rocSPARSE CSR SpMV, adaptive", "us": round(us, 2),
+            "GB/s": round(nbytes / us / 1e3, 1)}
+
+
+def bench_spmm(args):
+    """C = A B on the 1M-row banded matrix (offsets -3..3 plus 8 random
+    columns per row), B row-major with --n-dense columns: GB/s over
+    (n+1)*4 + nnz*(4 + s + N*s) + n*N*s (every B row gathered)."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, ROOT)
+    from flagsparse_public_amd import (flagsparse_spmm_csr_opt, prepare_spmm_csr_opt, synthetic,
+                                       vendor)
+    dev = torch.device("cuda", 0)
+    npd = np.float32 if args.dtype == "f32" else np.float64
+    ip, cols, vals = synthetic.spmm_banded_1m(dtype=npd)
+    n, nnz, N = ip.size - 1, int(ip[-1]), args.n_dense
+    s_v = np.dtype(npd).itemsize
+    d = [torch.from_numpy(a).to(dev) for a in (vals, cols, ip.astype(np.int32))]
+    B = torch.from_numpy(synthetic.dense_uniform(303, (n, N), npd)).to(dev)
+    p = prepare_spmm_csr_opt(*d, (n, n))
+    stream = torch.cuda.current_stream(dev)
+    us = _event_time(lambda: flagsparse_spmm_csr_opt(B=B, prepared=p), args.steps, stream)
+    nbytes = (n + 1) * 4 + nnz * (4 + s_v + N * s_v) + n * N * s_v
+    gbs = nbytes / us / 1e3
+    res = {"metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": 3, "ms_per_step": round(us / 1e3, 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+           "data": "synthetic",
+           "config": {"workload": "spmm_banded_1m", "n_rows": n, "nnz": nnz, "n_dense": N,
+                      "gflops": round(2.0 * nnz * N / us / 1e3, 2),
+                      "compulsory_bytes": (n + 1) * 4 + nnz * (4 + s_v) + 2 * n * N * s_v,
+                      "schedule": repr(p.schedule)},
+           "roofline": {"bound": "hbm", "kernel": "spmm_csr_kernel", "achieved": round(gbs, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "traffic": None, "algorithmic_bytes_per_launch": nbytes,
+                        "avg_launch_us": round(us, 3)}}
+    if not args.no_vendor and vendor.available():
+        C = torch.empty((n, N), dtype=B.dtype, device=dev)
+        v = vendor.VendorSpmm(*d, (n, n), B, C)
+        vus = _event_time(v.run, args.steps, stream)
+        res["vendor_baseline"] = {"library": "rocSPARSE CSR SpMM", "us": round(vus, 2),
+                                  "GB/s": round(nbytes / vus / 1e3, 1)}
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_batched(args):
+    """512 uneven CSR matrices, each with one SpMV and one 8-column SpMM, in
+    one persistent launch replayed from a HIP graph; beside it the 1,024
+    separate launches.  GB/s over the batch's algorithmic bytes."""
+    import torch
+    sys.path.insert(0, ROOT)
+    from flagsparse_public_amd.autotune import time_launches
+    from flagsparse_public_amd.batched import BatchedCsr
+    from flagsparse_public_amd.batched_workload import uneven_batch
+    from flagsparse_public_amd import prepare_spmm_csr_opt, prepare_spmv_csr
+    from flagsparse_public_amd.sparse_operations.spmm_csr import _hip_spmm_csr_impl_opt_prepared
+    from flagsparse_public_amd.sparse_operations.spmv_csr import _hip_spmv_csr_impl_opt_prepared
+    dev = torch.device("cuda", 0)
+    mats, meta = uneven_batch(512, 4, dev)
+    bat = BatchedCsr(mats)
+    graph = bat.capture()
+    us_graph = time_launches(graph.replay, reps=max(args.steps // 20, 5), warmup=2)
+    us_eager = time_launches(bat.run, reps=max(args.steps // 20, 5), warmup=2)
+    pv = [prepare_spmv_csr(m["data"], m["indices"], m["indptr"], m["shape"]) for m in mats]
+    pm = [prepare_spmm_csr_opt(m["data"], m["indices"], m["indptr"], m["shape"]) for m in mats]
+    ys = [torch.empty(m["shape"][0], device=dev) for m in mats]
+    Cs = [torch.empty((m["shape"][0], 8), device=dev) for m in mats]
+
+    def separate():
+        for m, a, b, y, C in zip(mats, pv, pm, ys, Cs):
+            _hip_spmv_csr_impl_opt_prepared(a, m["x"], out=y)
+            _hip_spmm_csr_impl_opt_prepared(b, m["B"], out=C)
+    us_sep = time_launches(separate, reps=3, warmup=1)
+    gbs = bat.bytes / us_graph / 1e3
+    res = {"metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
+           "steps": max(args.steps // 20, 5), "warmup": 2, "ms_per_step": round(us_graph / 1e3, 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "batched_512", **{k: v for k, v in meta.items()
+                                                   if isinstance(v, (int, float, str))},
+                      "eager_ms": round(us_eager / 1e3, 4),
+                      "separate_launches_ms": round(us_sep / 1e3, 4),
+                      "speedup_vs_separate": round(us_sep / us_graph, 3)},
+           "roofline": {"bound": "hbm", "kernel": "batched_kernel", "achieved": round(gbs, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "traffic": None, "algorithmic_bytes_per_launch": bat.bytes,
+                        "avg_launch_us": round(us_graph, 3)}}
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def calibrate_copy(mib, reps):
     """Plain device-to-device copies of a known size: the reference point that
     turns rocprofv3 FETCH_SIZE / WRITE_SIZE readings into bytes."""
@@ -151,6 +270,13 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-modes", action="store_true",
                     help="skip the split-precision and default-path lines")
+    ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE line")
+    ap.add_argument("--workload", default="spmv_powerlaw_4m",
+                    choices=["spmv_powerlaw_4m", "spmm_banded_1m", "batched_512"],
+                    help="the SpMV headline workload (default), the banded SpMM or the batch")
+    ap.add_argument("--n-dense", type=int, default=64, help="spmm_banded_1m: columns of B")
+    ap.add_argument("--dtype", choices=["f32", "f64"], default="f32",
+                    help="spmm_banded_1m: value type")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run device copies of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -160,6 +286,10 @@ def main(argv=None):
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(the SpMV row-block sharding is described, not run)", file=sys.stderr)
         return 2
+    if args.workload == "spmm_banded_1m":
+        return bench_spmm(args)
+    if args.workload == "batched_512":
+        return bench_batched(args)
 
     import numpy as np
     import torch
@@ -279,6 +409,8 @@ def main(argv=None):
             "step_frac": round(gbs / HBM_PEAK_GBS, 4),
         },
     }
+    if not args.no_vendor:
+        result["vendor_baseline"] = vendor_spmv(d_vals, d_cols, d_ip, d_x, n, nbytes, args.steps)
     modes = None
     if not args.no_modes:
         modes = measure_modes(d_vals, d_cols, d_ip, d_x, n, nnz, prepared, use_opt,
