@@ -9,6 +9,7 @@ __version__ = "0.1.0"
 
 _OPS_EXPORTS = {
     "PreparedCoo",
+    "SDDMMPrepared",
     "PreparedCsrSpmv",
     "PreparedCsrSpmmOpt",
     "PreparedCsrSpmmOptAlg2",
@@ -19,6 +20,8 @@ _OPS_EXPORTS = {
     "benchmark_spmm_opt_case",
     "benchmark_spmv_case",
     "comprehensive_spmm_test",
+    "benchmark_sddmm_case",
+    "flagsparse_sddmm_csr",
     "flagsparse_spmm_coo",
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
@@ -26,6 +29,7 @@ _OPS_EXPORTS = {
     "flagsparse_spmv_coo",
     "flagsparse_spmv_coo_tocsr",
     "flagsparse_spmv_csr",
+    "prepare_sddmm_csr",
     "prepare_spmm_csr_opt",
     "prepare_spmm_csr_opt_alg2",
     "prepare_spmv_coo",

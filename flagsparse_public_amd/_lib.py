@@ -178,6 +178,10 @@ _SIGNATURES = {
                                                             ctypes.c_double, ctypes.c_int64,
                                                             ctypes.c_void_p]),
     "fs_mlp_select": (ctypes.c_int, [ctypes.POINTER(FsMlp)] + [ctypes.c_void_p] * 6),
+    "fs_sddmm_csr": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                    ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_void_p]),
     "fs_mtx_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(FsMtxInfo),
                                    ctypes.POINTER(ctypes.c_void_p)]),
     "fs_mtx_fill": (ctypes.c_int, [ctypes.c_void_p] * 4),
@@ -392,6 +396,15 @@ def partials_bytes(value_type: int, accumulate: int, n_chunks: int, n_dense: int
 def sliced_combine(sl: FsSlices, z, y, alpha: float, beta: float, device) -> None:
     check(load().fs_sliced_combine(ctypes.byref(sl), ptr(z), ptr(y), float(alpha), float(beta),
                                    stream_handle(device)), "fs_sliced_combine")
+
+
+def sddmm(csr: FsCsr, row_ids, x, y, k_dim: int, out, out_type: int, accumulate: int,
+          alpha: float, beta: float, device) -> None:
+    lib = load()
+    with on_device(device):
+        check(lib.fs_sddmm_csr(ctypes.byref(csr), ptr(row_ids), ptr(x), ptr(y), int(k_dim),
+                               ptr(out), int(out_type), int(accumulate), float(alpha),
+                               float(beta), stream_handle(device)), "fs_sddmm_csr")
 
 
 def tile_carry_bytes(value_type: int, n_tiles: int) -> int:

@@ -404,102 +404,139 @@ __device__ __forceinline__ void entry_finish(const SpmvParams<VS, V, I, P>& p, i
   else entry_write(p, row, acc);
 }
 
-template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>
-__global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I, P> p) {
-  using C = typename A::compute_t;
-  using S = typename TileSum<C>::T;
+// One entry tile held in registers between its loads and its LDS stage.
+template <typename I, typename C, int E> struct EntryTile {
+  static constexpr int kPv = kSegRows / kBlock + 1;
+  I col[E];
+  C a[E];
+  int64_t pv[kPv];  // first batch of row pointers (i <= nb0)
+  int64_t t, e0, r0, r1;
+  int len, nb0;
+  bool vec;  // loaded with the 4-entry vector layout
+};
+
+// Bounds of tile t and every load that does not depend on another load:
+// the col / value entries (16-byte vector loads when the tile is full and
+// aligned, else scalar loads masked by the tile length) and the first batch
+// of row pointers.
+template <int E, bool VEC, typename VS, typename V, typename I, typename P, typename C>
+__device__ __forceinline__ void entry_fetch(const SpmvParams<VS, V, I, P>& p, int64_t t,
+                                            EntryTile<I, C, E>& f) {
   constexpr int T = E * kBlock;
-  constexpr int kPv = kSegRows / kBlock + 1;
-  extern __shared__ __align__(16) char smem[];
-  C* prod = reinterpret_cast<C*>(smem);
-  int32_t* bnd = reinterpret_cast<int32_t*>(smem + sizeof(C) * T);
-  S* red = reinterpret_cast<S*>(smem + sizeof(C) * T + sizeof(int32_t) * (kSegRows + 4));
-  S* carries = reinterpret_cast<S*>(p.partials);
-  const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
-  int64_t t = blockIdx.x;
-  if (p.n_groups > 0) {  // block b serves group b % G, in order within the group
-    const int g = (int)(t % p.n_groups);
-    t = p.group_tiles[g] + t / p.n_groups;
-    if (t >= p.group_tiles[g + 1] || t >= p.n_tiles) return;
-  }
+  const int tid = threadIdx.x;
+  f.t = t;
   // entry range [e0, e0 + len): len <= T (clamped, so a damaged plan cannot
   // overrun the LDS products)
-  const int64_t e0 = p.tile_entry[t];
+  f.e0 = p.tile_entry[t];
   int64_t e1 = p.tile_entry[t + 1];
   e1 = e1 > p.nnz ? p.nnz : e1;
-  const int64_t span = e1 - e0;
-  const int len = span <= 0 ? 0 : (span < T ? (int)span : T);
+  const int64_t span = e1 - f.e0;
+  f.len = span <= 0 ? 0 : (span < T ? (int)span : T);
   // owned rows [r0, r1), clamped to [0, n_rows] so a damaged plan cannot
   // send a row-pointer read out of bounds
   int64_t r0 = p.tile_start[t], r1 = p.tile_start[t + 1];
   r0 = r0 < 0 ? 0 : (r0 > p.n_rows ? p.n_rows : r0);
   r1 = r1 < r0 ? r0 : (r1 > p.n_rows ? p.n_rows : r1);
-  const int nb0 = (int)(r1 - r0 < kSegRows ? r1 - r0 : kSegRows);
-  // first batch of row pointers (i <= nb0), loaded beside the entries
-  int64_t pv[kPv];
+  f.r0 = r0;
+  f.r1 = r1;
+  f.nb0 = (int)(r1 - r0 < kSegRows ? r1 - r0 : kSegRows);
 #pragma unroll
-  for (int k = 0; k < kPv; ++k) {
+  for (int k = 0; k < EntryTile<I, C, E>::kPv; ++k) {
     const int i = k * kBlock + tid;
-    pv[k] = i <= nb0 ? (int64_t)p.ptr[r0 + i] : 0;
+    f.pv[k] = i <= f.nb0 ? (int64_t)p.ptr[r0 + i] : 0;
   }
-  if (len == T && (!VEC || (e0 & 3) == 0)) {
-    I col[E];
-    C a[E];
+  f.vec = VEC && f.len == T && (f.e0 & 3) == 0;
+  if (f.vec) {
     if constexpr (VEC) {
 #pragma unroll
       for (int g = 0; g < E / 4; ++g) {
-        const int64_t j = e0 + (int64_t)(g * kBlock + tid) * 4;
+        const int64_t j = f.e0 + (int64_t)(g * kBlock + tid) * 4;
         I ci[4];
         load4(p.idx, j, ci);
         C av[4];
         p.vals.template get4<C>(j, av);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          col[4 * g + k] = ci[k];
-          a[4 * g + k] = av[k];
+          f.col[4 * g + k] = ci[k];
+          f.a[4 * g + k] = av[k];
         }
       }
-    } else {
-#pragma unroll
-      for (int u = 0; u < E; ++u) {
-        const int64_t j = e0 + u * kBlock + tid;
-        col[u] = p.idx[j];
-        a[u] = p.vals.template get<C>(j);
-      }
     }
-    C xv[E];
+  } else if (f.len == T) {
 #pragma unroll
-    for (int u = 0; u < E; ++u) xv[u] = load_as<C>(p.x, (int64_t)col[u]);
-    if constexpr (VEC) {
-#pragma unroll
-      for (int g = 0; g < E / 4; ++g)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) prod[(g * kBlock + tid) * 4 + k] = mulc(a[4 * g + k], xv[4 * g + k]);
-    } else {
-#pragma unroll
-      for (int u = 0; u < E; ++u) prod[u * kBlock + tid] = mulc(a[u], xv[u]);
+    for (int u = 0; u < E; ++u) {
+      const int64_t j = f.e0 + u * kBlock + tid;
+      f.col[u] = p.idx[j];
+      f.a[u] = p.vals.template get<C>(j);
     }
   } else {
-    // the last, partial tile: scalar loads with bounds
 #pragma unroll
     for (int u = 0; u < E; ++u) {
       const int o = u * kBlock + tid;
-      if (o < len) {
-        const int64_t j = e0 + o;
-        prod[o] = mulc(p.vals.template get<C>(j), load_as<C>(p.x, ldg_index(p.idx, j)));
-      }
+      const int64_t j = f.e0 + o;
+      f.col[u] = o < f.len ? p.idx[j] : I(0);
+      f.a[u] = o < f.len ? p.vals.template get<C>(j) : zero<C>();
+    }
+  }
+}
+
+// The x gathers of a fetched tile (masked past its length).
+template <int E, typename VS, typename V, typename I, typename P, typename C>
+__device__ __forceinline__ void entry_gather(const SpmvParams<VS, V, I, P>& p,
+                                             const EntryTile<I, C, E>& f, C (&xv)[E]) {
+  constexpr int T = E * kBlock;
+  if (f.len == T) {
+#pragma unroll
+    for (int u = 0; u < E; ++u) xv[u] = load_as<C>(p.x, (int64_t)f.col[u]);
+  } else {  // a partial tile is never in the vector layout
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int o = u * kBlock + tid;
+      xv[u] = o < f.len ? load_as<C>(p.x, (int64_t)f.col[u]) : zero<C>();
+    }
+  }
+}
+
+// Products to LDS in entry order, and the tile's row boundaries (entry
+// offsets clamped to [0, len]) of its first batch of owned rows.
+template <int E, typename I, typename C>
+__device__ __forceinline__ void entry_stage(const EntryTile<I, C, E>& f, const C (&xv)[E], C* prod,
+                                            int32_t* bnd) {
+  const int tid = threadIdx.x;
+  if (f.vec) {
+#pragma unroll
+    for (int g = 0; g < E / 4; ++g)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) prod[(g * kBlock + tid) * 4 + k] = mulc(f.a[4 * g + k], xv[4 * g + k]);
+  } else {
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int o = u * kBlock + tid;
+      if (o < f.len) prod[o] = mulc(f.a[u], xv[u]);
     }
   }
 #pragma unroll
-  for (int k = 0; k < kPv; ++k) {
+  for (int k = 0; k < EntryTile<I, C, E>::kPv; ++k) {
     const int i = k * kBlock + tid;
-    if (i <= nb0) {
-      const int64_t v = pv[k] - e0;
-      bnd[i] = (int)(v < 0 ? 0 : (v > len ? len : v));
+    if (i <= f.nb0) {
+      const int64_t v = f.pv[k] - f.e0;
+      bnd[i] = (int)(v < 0 ? 0 : (v > f.len ? f.len : v));
     }
   }
+}
+
+// After the LDS stage and a barrier: the head carry (entries of a row owned
+// by an earlier tile), then every owned row summed in fp64 in entry order,
+// a row running past the tile going to the tail carry.
+template <typename A, int E, typename VS, typename V, typename I, typename P, typename C, typename S>
+__device__ __forceinline__ void entry_rows(const SpmvParams<VS, V, I, P>& p,
+                                           const EntryTile<I, C, E>& f, const C* prod,
+                                           int32_t* bnd, S* red, S* carries) {
+  const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
+  const int64_t t = f.t, e0 = f.e0, r0 = f.r0, r1 = f.r1;
+  const int len = f.len;
   const bool straddle = r1 > r0 && (int64_t)p.ptr[r1] > e0 + len;
-  __syncthreads();
   // head: entries [0, bnd[0]) belong to a row owned by an earlier tile
   const int head = bnd[0];
   if (head > 0 && head <= 4 * kWave) {
@@ -554,6 +591,42 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I,
       if (lane == 0) entry_finish<A>(p, row, acc.value(), straddle && row == r1 - 1, carries, t);
     }
   }
+}
+
+// LDS carve-up shared by the entry kernels: products, row boundaries, the
+// per-wavefront reduction slots.
+template <typename C, typename S, int T> struct EntryLds {
+  C* prod;
+  int32_t* bnd;
+  S* red;
+  __device__ __forceinline__ explicit EntryLds(char* smem)
+      : prod(reinterpret_cast<C*>(smem)),
+        bnd(reinterpret_cast<int32_t*>(smem + sizeof(C) * T)),
+        red(reinterpret_cast<S*>(smem + sizeof(C) * T + sizeof(int32_t) * (kSegRows + 4))) {}
+};
+
+// One tile per block (block b takes tile b, or with block groups the b / G-th
+// tile of group b % G).
+template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>
+__global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I, P> p) {
+  using C = typename A::compute_t;
+  using S = typename TileSum<C>::T;
+  extern __shared__ __align__(16) char smem[];
+  const EntryLds<C, S, E * kBlock> lds(smem);
+  S* carries = reinterpret_cast<S*>(p.partials);
+  int64_t t = blockIdx.x;
+  if (p.n_groups > 0) {  // block b serves group b % G, in order within the group
+    const int g = (int)(t % p.n_groups);
+    t = p.group_tiles[g] + t / p.n_groups;
+    if (t >= p.group_tiles[g + 1] || t >= p.n_tiles) return;
+  }
+  EntryTile<I, C, E> f;
+  entry_fetch<E, VEC>(p, t, f);
+  C xv[E];
+  entry_gather(p, f, xv);
+  entry_stage(f, xv, lds.prod, lds.bnd);
+  __syncthreads();
+  entry_rows<A>(p, f, lds.prod, lds.bnd, lds.red, carries);
 }
 
 // Rows cut by a tile boundary: thread t takes tile t's last row when it runs

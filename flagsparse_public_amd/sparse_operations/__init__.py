@@ -24,9 +24,12 @@ from .spmm_csr_opt_alg2 import (
 from .benchmarks import benchmark_spmm_case, benchmark_spmv_case, comprehensive_spmm_test
 from .spmv_coo import PreparedCoo, flagsparse_spmv_coo, prepare_spmv_coo
 from .spmm_coo import flagsparse_spmm_coo
+from .sddmm_csr import (SDDMMPrepared, benchmark_sddmm_case, flagsparse_sddmm_csr,
+                         prepare_sddmm_csr)
 
 __all__ = [
     "PreparedCoo",
+    "SDDMMPrepared",
     "PreparedCsrSpmv",
     "PreparedCsrSpmmOpt",
     "PreparedCsrSpmmOptAlg2",
@@ -37,6 +40,8 @@ __all__ = [
     "benchmark_spmm_opt_case",
     "benchmark_spmv_case",
     "comprehensive_spmm_test",
+    "benchmark_sddmm_case",
+    "flagsparse_sddmm_csr",
     "flagsparse_spmm_coo",
     "flagsparse_spmm_csr",
     "flagsparse_spmm_csr_opt",
@@ -44,6 +49,7 @@ __all__ = [
     "flagsparse_spmv_coo",
     "flagsparse_spmv_coo_tocsr",
     "flagsparse_spmv_csr",
+    "prepare_sddmm_csr",
     "prepare_spmm_csr_opt",
     "prepare_spmm_csr_opt_alg2",
     "prepare_spmv_coo",

@@ -160,6 +160,20 @@ enum { FS_TILE_ROWS = 0, FS_TILE_ENTRIES = 1 };
 /* Bytes of `partials` for FS_TILE_ENTRIES with n_tiles tiles. */
 int64_t fs_tile_carry_bytes(int32_t value_type, int64_t n_tiles);
 
+/* --------------------------------------------------------------- SDDMM */
+/*
+ * out[p] = alpha * sum_k x[row(p), k] * y[col(p), k] + beta * data[p] for every
+ * entry p of the CSR pattern P (P->values = data, or null for beta = 0).
+ * Replaces the reference's Triton SDDMM (sddmm_csr.py:108-151, launched by
+ * _run_sddmm_prepared :257-330).  x is n_rows x k_dim and y n_cols x k_dim,
+ * row-major contiguous, of P->value_type (FS_F32 / FS_F64); row_ids[p] is the
+ * row of entry p (int32); columns int32.  accumulate: FS_ACC_NATIVE sums in
+ * the value type, FS_ACC_WIDE in fp64; out_type: the value type or FS_F64.
+ */
+int fs_sddmm_csr(const fs_csr *P, const int32_t *row_ids, const void *x, const void *y,
+                 int64_t k_dim, void *out, int32_t out_type, int32_t accumulate, double alpha,
+                 double beta, void *stream);
+
 /* ------------------------------------------------ MatrixMarket input */
 /*
  * Replaces the reference harness's Python loader load_mtx_to_csr_torch

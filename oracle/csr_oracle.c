@@ -94,6 +94,37 @@ void orc_spmm_f64(int64_t n_rows, int64_t n_dense, const int64_t *indptr,
     }
 }
 
+/* ----------------------------------------------------------------- SDDMM */
+
+/* out[p] = alpha * sum_k x[row(p), k] * y[indices[p], k] + beta * data[p]
+ * (data may be NULL: then no beta term), fp64, k in increasing order;
+ * follows sddmm_csr.py:108-151 (per entry: dot of an x row and a y row) and
+ * its torch reference sddmm_csr.py:423-437.  Also the condition
+ * abs_out[p] = |alpha| * sum_k |x[row(p), k] * y[indices[p], k]| + |beta * data[p]|. */
+void orc_sddmm_f64(int64_t n_rows, int64_t k_dim, const int64_t *indptr, const int64_t *indices,
+                   const double *x, const double *y, const double *data, double alpha,
+                   double beta, double *out, double *abs_out)
+{
+    for (int64_t r = 0; r < n_rows; ++r) {
+        const double *xr = x + r * k_dim;
+        for (int64_t p = indptr[r]; p < indptr[r + 1]; ++p) {
+            const double *yc = y + indices[p] * k_dim;
+            double s = 0.0, a = 0.0;
+            for (int64_t k = 0; k < k_dim; ++k) {
+                s += xr[k] * yc[k];
+                a += fabs(xr[k] * yc[k]);
+            }
+            double v = alpha * s, c = fabs(alpha) * a;
+            if (data) {
+                v += beta * data[p];
+                c += fabs(beta * data[p]);
+            }
+            out[p] = v;
+            if (abs_out) abs_out[p] = c;
+        }
+    }
+}
+
 void orc_spmm_abs_f64(int64_t n_rows, int64_t n_dense, const int64_t *indptr,
                       const int64_t *indices, const double *data, const double *B,
                       int64_t ldb, double *S, int64_t lds)

@@ -50,6 +50,7 @@ def lib():
             "orc_spmm_f64": (None, [i64, i64, vp, vp, vp, vp, i64, vp, i64]),
             "orc_spmm_abs_f64": (None, [i64, i64, vp, vp, vp, vp, i64, vp, i64]),
             "orc_spmm_c128": (None, [i64, i64, vp, vp, vp, vp, i64, vp, i64]),
+            "orc_sddmm_f64": (None, [i64, i64, vp, vp, vp, vp, vp, f64, f64, vp, vp]),
             "orc_csr_transpose": (None, [i64, i64, vp, vp, vp, vp, vp]),
             "orc_spmv_opt_bucket": (ctypes.c_int, [i64, ctypes.c_int]),
             "orc_spmm_opt_bucket": (ctypes.c_int, [i64]),
@@ -144,6 +145,23 @@ def spmm(indptr, indices, data, B) -> np.ndarray:
     C = np.zeros((n, nd), dtype=np.float64)
     lib().orc_spmm_f64(n, nd, _p(indptr), _p(indices), _p(d), _p(Bf), nd, _p(C), nd)
     return C
+
+
+def sddmm(indptr, indices, x, y, data=None, alpha=1.0, beta=0.0):
+    """(out, abs_out) of SDDMM in fp64 (sddmm_csr.py:108-151, 423-437)."""
+    indptr, indices = _i64(indptr), _i64(indices)
+    n = indptr.size - 1
+    xf = np.ascontiguousarray(x, dtype=np.float64)
+    yf = np.ascontiguousarray(y, dtype=np.float64)
+    k = xf.shape[1]
+    nnz = indices.size
+    out = np.zeros(nnz)
+    ab = np.zeros(nnz)
+    d = _f64(data) if data is not None else None
+    lib().orc_sddmm_f64(n, k, _p(indptr), _p(indices), _p(xf), _p(yf),
+                        _p(d) if d is not None else None, float(alpha), float(beta), _p(out),
+                        _p(ab))
+    return out, ab
 
 
 def spmm_abs(indptr, indices, data, B) -> np.ndarray:
