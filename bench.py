@@ -164,6 +164,7 @@ def bench_spmm(args):
     sys.path.insert(0, ROOT)
     from flagsparse_public_amd import (flagsparse_spmm_csr_opt, prepare_spmm_csr_opt, synthetic,
                                        vendor)
+    from flagsparse_public_amd.sparse_operations.spmm_csr import spmm_launch_schedule
     dev = torch.device("cuda", 0)
     npd = np.float32 if args.dtype == "f32" else np.float64
     ip, cols, vals = synthetic.spmm_banded_1m(dtype=npd)
@@ -172,6 +173,7 @@ def bench_spmm(args):
     d = [torch.from_numpy(a).to(dev) for a in (vals, cols, ip.astype(np.int32))]
     B = torch.from_numpy(synthetic.dense_uniform(303, (n, N), npd)).to(dev)
     p = prepare_spmm_csr_opt(*d, (n, n))
+    launch = spmm_launch_schedule(p, N)
     stream = torch.cuda.current_stream(dev)
     us = _event_time(lambda: flagsparse_spmm_csr_opt(B=B, prepared=p), args.steps, stream)
     nbytes = (n + 1) * 4 + nnz * (4 + s_v + N * s_v) + n * N * s_v
@@ -183,7 +185,7 @@ def bench_spmm(args):
            "config": {"workload": "spmm_banded_1m", "n_rows": n, "nnz": nnz, "n_dense": N,
                       "gflops": round(2.0 * nnz * N / us / 1e3, 2),
                       "compulsory_bytes": (n + 1) * 4 + nnz * (4 + s_v) + 2 * n * N * s_v,
-                      "schedule": repr(p.schedule)},
+                      "schedule": repr(launch[0]), "col_tile": launch[1]},
            "roofline": {"bound": "hbm", "kernel": "spmm_csr_kernel", "achieved": round(gbs, 2),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                         "traffic": None, "algorithmic_bytes_per_launch": nbytes,

@@ -123,13 +123,32 @@ __device__ __forceinline__ void mm_accumulate(A (&acc)[VW], const I* __restrict_
 #pragma unroll
       for (int k = 0; k < VW; ++k) acc[k].add_prod(a[u], b[u][k]);
   }
-  for (; j < e; j += step) {
-    const int64_t c = ldg_index(idx, j);
-    const C a = load_as<C>(val, j);
-    C b[VW];
-    load_vec<C, VW>(b, B, c * ldb + n0);
+  if (j < e) {
+    // the last, partial batch: all its loads in flight at once, entries past
+    // e masked (their products are not added)
+    int64_t col[U];
+    C a[U];
 #pragma unroll
-    for (int k = 0; k < VW; ++k) acc[k].add_prod(a, b[k]);
+    for (int u = 0; u < U; ++u) {
+      const bool ok = j + u * step < e;
+      col[u] = ok ? ldg_index(idx, j + u * step) : 0;
+      a[u] = ok ? load_as<C>(val, j + u * step) : zero<C>();
+    }
+    C b[U][VW];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (j + u * step < e) {
+        load_vec<C, VW>(b[u], B, col[u] * ldb + n0);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) b[u][k] = zero<C>();
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j + u * step < e)
+#pragma unroll
+        for (int k = 0; k < VW; ++k) acc[k].add_prod(a[u], b[u][k]);
   }
 }
 
@@ -250,7 +269,17 @@ int launch_spmm_gv(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t
   if (grid == 0) return FS_OK;
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmm_csr: grid of %lld blocks too large", (long long)grid);
-  spmm_csr_kernel<V, A, I, P, G, VW, 4><<<dim3((unsigned)grid), dim3(kMmBlock), 0, st>>>(p);
+  // entries in flight per lane: 8 on request for fp32 / fp64 with int32
+  // columns (short rows then finish in one or two batches), else 4
+  bool launched = false;
+  if constexpr (kHotSpmm<V, I>) {
+    if (pl->unroll >= 8) {
+      spmm_csr_kernel<V, A, I, P, G, VW, 8><<<dim3((unsigned)grid), dim3(kMmBlock), 0, st>>>(p);
+      launched = true;
+    }
+  }
+  if (!launched)
+    spmm_csr_kernel<V, A, I, P, G, VW, 4><<<dim3((unsigned)grid), dim3(kMmBlock), 0, st>>>(p);
   int rc = check_launch("fs_spmm_csr");
   if (rc || pl->n_chunks <= pl->n_long) return rc;  // no row spans several chunks
   const int64_t blocks = (pl->n_long + kMmWaves - 1) / kMmWaves;
@@ -301,7 +330,8 @@ int launch_spmm(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t ld
       }
     }
   }
-  int64_t tile = col_tile > 0 ? col_tile : n_dense;
+  // a column tile wider than B would leave the lanes past n_dense idle
+  int64_t tile = col_tile > 0 && col_tile < n_dense ? col_tile : n_dense;
   if (tile % vw != 0) vw = 1;
   int G = pow2_at_least((tile + vw - 1) / vw);
   if (vw == 4) return launch_spmm_g<V, A, I, P, 4>(G, M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);

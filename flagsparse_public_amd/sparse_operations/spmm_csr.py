@@ -13,6 +13,7 @@ compensated summation).  Extensions: trailing ``alpha`` / ``beta``.
 
 from __future__ import annotations
 
+import dataclasses
 import time
 
 import torch
@@ -318,12 +319,13 @@ class PreparedCsrSpmmOpt:
         "data", "kernel_indices", "kernel_indptr", "shape", "n_rows", "n_cols", "row_lengths",
         "max_row_nnz", "row_buckets", "supports_opt", "long_part_rows", "long_part_starts",
         "long_part_ends", "long_row_ids", "long_row_part_ptr", "long_row_fallback_only",
-        "schedule", "_plans", "_csr",
+        "schedule", "library_schedule", "_plans", "_csr",
     )
 
     def __init__(self, data, kernel_indices, kernel_indptr, shape, n_rows, n_cols, row_lengths,
                  max_row_nnz, row_buckets, long_part_rows, long_part_starts, long_part_ends,
-                 long_row_ids, long_row_part_ptr, long_row_fallback_only, schedule=None):
+                 long_row_ids, long_row_part_ptr, long_row_fallback_only, schedule=None,
+                 library_schedule=False):
         self.data = data
         self.kernel_indices = kernel_indices
         self.kernel_indptr = kernel_indptr
@@ -341,6 +343,7 @@ class PreparedCsrSpmmOpt:
         self.long_row_part_ptr = long_row_part_ptr
         self.long_row_fallback_only = bool(long_row_fallback_only)
         self.schedule = schedule
+        self.library_schedule = bool(library_schedule)  # chosen by the library, not the caller
         self._plans = {}
         self._csr = None
 
@@ -427,6 +430,7 @@ def prepare_spmm_csr_opt(data, indices, indptr, shape, schedule=None):
     stats_out = []
     (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
      max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape, _stats_out=stats_out)
+    library = not isinstance(schedule, Schedule)
     schedule, _ = choose_schedule("spmm64", schedule, data, n_rows, n_cols, stats_out[0])
     row_buckets, long_rows = _build_spmm_opt_buckets(row_lengths)
     part_rows, part_starts, part_ends, part_ptr = _build_spmm_opt_split_metadata(
@@ -436,7 +440,7 @@ def prepare_spmm_csr_opt(data, indices, indptr, shape, schedule=None):
         n_rows=n_rows, n_cols=n_cols, row_lengths=row_lengths, max_row_nnz=max_row_nnz,
         row_buckets=row_buckets, long_part_rows=part_rows, long_part_starts=part_starts,
         long_part_ends=part_ends, long_row_ids=long_rows, long_row_part_ptr=part_ptr,
-        long_row_fallback_only=False, schedule=schedule)
+        long_row_fallback_only=False, schedule=schedule, library_schedule=library)
 
 
 def _check_opt_B(prepared, B):
@@ -454,14 +458,44 @@ def _check_opt_B(prepared, B):
         raise ValueError("B must be a 2D dense tensor")
 
 
+def _library_spmm_launch(sch, dtype, n_dense):
+    """(schedule, column tile) to launch for a schedule the library chose
+    itself (selector, fixed rule or exhaustive search).  Those are picked for
+    B with 64 columns at 4 entries in flight per lane; measured on MI355X
+    (1M-row banded matrix, N = 32 / 64 / 128), fp32 runs faster with 8 in
+    flight (418 vs 437 us at N = 64), and a wider B keeps the chosen lanes per
+    64 columns (N = 128 with one 128-column tile: 883 vs 926 us).  fp64 keeps
+    4 in flight (851 vs 929 us).  Schedules passed in by the caller run as
+    given."""
+    if dtype == torch.float32 and sch.unroll == 4:
+        sch = dataclasses.replace(sch, unroll=8)
+    col_tile = sch.col_tile
+    if col_tile > 0 and n_dense > 64:
+        col_tile *= n_dense // 64
+    return sch, col_tile
+
+
+def spmm_launch_schedule(prepared, n_dense, schedule=None):
+    """The (schedule, column tile) an opt / alg2 SpMM call with n_dense
+    columns launches."""
+    if schedule is None and prepared.library_schedule:
+        sch = prepared.schedule or default_spmm_schedule(prepared.n_rows, prepared.data.numel(),
+                                                         n_dense)
+        return _library_spmm_launch(sch, prepared.data.dtype, n_dense)
+    sch = schedule or prepared.schedule or default_spmm_schedule(prepared.n_rows,
+                                                                 prepared.data.numel(), n_dense)
+    return sch, sch.col_tile
+
+
 def _hip_spmm_csr_impl_opt_prepared(prepared, B, out=None, alpha=1.0, beta=0.0, schedule=None):
     """Opt path (spmm_csr.py:2034-2056): native accumulation; returns (C, False)."""
     _check_opt_B(prepared, B)
     B = B.contiguous()
     n_dense = int(B.shape[1])
-    plan = prepared.plan(_lib.FS_ACC_NATIVE, n_dense, schedule)
+    sch, col_tile = spmm_launch_schedule(prepared, n_dense, schedule)
+    plan = prepared.plan(_lib.FS_ACC_NATIVE, n_dense, sch)
     C = _launch_spmm(prepared.csr_struct(), plan, B, prepared.n_rows, n_dense, B.dtype, B.device,
-                     out, alpha, beta, plan.schedule.col_tile)
+                     out, alpha, beta, col_tile)
     return C, False
 
 

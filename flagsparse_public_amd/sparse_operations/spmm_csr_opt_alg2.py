@@ -15,7 +15,7 @@ import time
 import torch
 
 from .. import _lib
-from ..rowsplit import default_spmm_schedule
+from ..rowsplit import Schedule
 from ..schedule_choice import choose_schedule
 from ._common import _INDEX_LIMIT_INT32
 from .spmm_csr import (  # noqa: F401  (re-exported like the reference)
@@ -27,6 +27,7 @@ from .spmm_csr import (  # noqa: F401  (re-exported like the reference)
     flagsparse_spmm_csr,
     flagsparse_spmm_csr_opt,
     prepare_spmm_csr_opt,
+    spmm_launch_schedule,
 )
 
 SUPPORTED_SPMM_OPT_ALG2_DTYPES = (torch.float32, torch.float64)
@@ -37,10 +38,10 @@ class PreparedCsrSpmmOptAlg2:
 
     __slots__ = ("data", "kernel_indices", "kernel_indptr", "shape", "n_rows", "n_cols",
                  "row_lengths", "max_row_nnz", "opt_buckets", "supports_opt", "schedule",
-                 "_plans", "_csr")
+                 "library_schedule", "_plans", "_csr")
 
     def __init__(self, data, kernel_indices, kernel_indptr, shape, n_rows, n_cols, row_lengths,
-                 max_row_nnz, opt_buckets, schedule=None):
+                 max_row_nnz, opt_buckets, schedule=None, library_schedule=False):
         self.data = data
         self.kernel_indices = kernel_indices
         self.kernel_indptr = kernel_indptr
@@ -52,6 +53,7 @@ class PreparedCsrSpmmOptAlg2:
         self.opt_buckets = opt_buckets
         self.supports_opt = data.dtype in SUPPORTED_SPMM_OPT_ALG2_DTYPES
         self.schedule = schedule
+        self.library_schedule = bool(library_schedule)  # chosen by the library, not the caller
         self._plans = {}
         self._csr = None
 
@@ -210,11 +212,13 @@ def prepare_spmm_csr_opt_alg2(data, indices, indptr, shape, schedule=None):
     stats_out = []
     (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
      max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape, _stats_out=stats_out)
+    library = not isinstance(schedule, Schedule)
     schedule, _ = choose_schedule("spmm64", schedule, data, n_rows, n_cols, stats_out[0])
     return PreparedCsrSpmmOptAlg2(
         data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
         n_rows=n_rows, n_cols=n_cols, row_lengths=row_lengths, max_row_nnz=max_row_nnz,
-        opt_buckets=_build_spmm_opt_alg2_buckets(row_lengths, data.dtype), schedule=schedule)
+        opt_buckets=_build_spmm_opt_alg2_buckets(row_lengths, data.dtype), schedule=schedule,
+        library_schedule=library)
 
 
 def _validate_spmm_opt_alg2_runtime_inputs(prepared, B, out):
@@ -247,12 +251,11 @@ def _hip_spmm_csr_impl_opt_alg2_prepared(prepared, B, return_meta=False, out=Non
     """Run the native-accumulation SpMM; meta as spmm_csr_opt_alg2.py:771-798."""
     device_props = _normalize_spmm_opt_alg2_device_props(prepared.data.device)
     n_dense = int(B.shape[1])
-    sch = schedule or prepared.schedule or default_spmm_schedule(prepared.n_rows,
-                                                                 prepared.data.numel(), n_dense)
+    sch, col_tile = spmm_launch_schedule(prepared, n_dense, schedule)
     plan = _plan_for(prepared._plans, prepared.kernel_indptr, prepared.n_rows,
                      prepared.data.dtype, _lib.FS_ACC_NATIVE, sch, prepared.row_lengths)
     C = _launch_spmm(prepared.csr_struct(), plan, B, prepared.n_rows, n_dense, B.dtype, B.device,
-                     out, alpha, beta, sch.col_tile)
+                     out, alpha, beta, col_tile)
     meta = None
     if return_meta:
         launches = [_resolve_spmm_opt_alg2_launch(b, n_dense, prepared.data.dtype, device_props)
@@ -268,7 +271,7 @@ def _hip_spmm_csr_impl_opt_alg2_prepared(prepared, B, return_meta=False, out=Non
             "n_dense_cols": n_dense,
             "hip_schedule": {"short_max": sch.short_max, "chunk_nnz": sch.chunk_nnz,
                              "n_long_rows": plan.n_long, "n_chunks": plan.n_chunks,
-                             "col_tile": sch.col_tile},
+                             "col_tile": col_tile, "unroll": sch.unroll},
         }
     return C, meta
 

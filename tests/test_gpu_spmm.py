@@ -109,11 +109,14 @@ def test_spmm_default_path_grid(orc, dtype, n_dense):
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
-@pytest.mark.parametrize("n_dense", [1, 8, 32, 64, 96, 128])
+@pytest.mark.parametrize("n_dense", [1, 8, 32, 64, 96, 128, 200])
 @pytest.mark.parametrize("sch", [None, Schedule(short_max=16, chunk_nnz=64),
                                  Schedule(short_max=0, chunk_nnz=128, xcd_remap=1),
-                                 Schedule(short_max=64, chunk_nnz=256, col_tile=32)],
-                         ids=["default", "chunky", "allchunk_xcd", "tile32"])
+                                 Schedule(short_max=64, chunk_nnz=256, col_tile=32),
+                                 Schedule(short_max=16, chunk_nnz=64, unroll=8),
+                                 Schedule(short_max=64, chunk_nnz=256, col_tile=64, unroll=8)],
+                         ids=["default", "chunky", "allchunk_xcd", "tile32", "chunky_u8",
+                              "tile64_u8"])
 def test_spmm_opt_and_alg2_schedules(orc, dtype, n_dense, sch):
     rng = np.random.default_rng(100 + n_dense)
     npd = NP[dtype]
