@@ -111,6 +111,7 @@ class FsSlices(ctypes.Structure):
         ("bounds", ctypes.c_void_p),
         ("rows_per_block", ctypes.c_int32),
         ("z_type", ctypes.c_int32),
+        ("row_mask", ctypes.c_void_p),
     ]
 
 

@@ -68,14 +68,81 @@ __global__ __launch_bounds__(kCombineRows) void sliced_combine_kernel(
   if (row < n_rows) write_out(y, row, acc[tid], alpha, beta);
 }
 
+// The same sums from the row masks: thread t owns row 256 b + t and, slice
+// by slice, finds its partial (if any) at bounds[k][b] + the number of rows
+// of the block before it that have one in slice k.  The mask words of kGroup
+// slices, their prefix bit counts and the bounds are staged in LDS; each
+// thread then issues its kGroup partial loads together and adds them in
+// slice order in registers: no row ids are read and no barriers separate
+// the slices.
+constexpr int kMaskWords = kCombineRows / 32;
+
+template <typename Z, typename V>
+__global__ __launch_bounds__(kCombineRows) void sliced_combine_mask_kernel(
+    const Z* __restrict__ z, const uint32_t* __restrict__ row_mask,
+    const int64_t* __restrict__ bounds, int32_t n_slices, int64_t n_rows, int64_t n_blocks,
+    int64_t n_compact, V* __restrict__ y, double alpha, double beta) {
+  using C = typename Native<Z>::T;
+  using S = typename TileSum<C>::T;
+  __shared__ uint32_t words[kGroup][kMaskWords];
+  __shared__ int32_t before[kGroup][kMaskWords];  // set bits in the words before j
+  __shared__ int64_t base[kGroup];
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int w = tid / 32, bit = tid % 32;
+  const uint32_t below = bit == 0 ? 0u : (0xffffffffu >> (32 - bit));
+  S acc = zero<S>();
+  for (int k0 = 0; k0 < n_slices; k0 += kGroup) {
+    if (k0 > 0) __syncthreads();  // the previous group's words are consumed
+    if (tid < kGroup * kMaskWords) {
+      const int u = tid / kMaskWords, j = tid % kMaskWords;
+      words[u][j] = k0 + u < n_slices ? row_mask[((int64_t)(k0 + u) * n_blocks + b) * kMaskWords + j]
+                                      : 0u;
+    } else if (tid >= 128 && tid < 128 + kGroup) {
+      const int u = tid - 128;
+      base[u] = k0 + u < n_slices ? bounds[(int64_t)(k0 + u) * (n_blocks + 1) + b] : 0;
+    }
+    __syncthreads();
+    if (tid < kGroup) {
+      int c = 0;
+#pragma unroll
+      for (int j = 0; j < kMaskWords; ++j) {
+        before[tid][j] = c;
+        c += __popc(words[tid][j]);
+      }
+    }
+    __syncthreads();
+    C zv[kGroup];
+    unsigned present = 0;  // slices of this group holding a partial of the row
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+      const uint32_t mine = words[u][w];
+      const int64_t i = base[u] + before[u][w] + __popc(mine & below);
+      const bool has = ((mine >> bit) & 1u) && i < n_compact;  // a bad mask reads nothing
+      zv[u] = has ? load_as<C>(z, i) : zero<C>();
+      present |= (unsigned)has << u;
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u)
+      if ((present >> u) & 1u) acc = add(acc, widen(zv[u]));
+  }
+  const int64_t row = b * kCombineRows + tid;
+  if (row < n_rows) write_out(y, row, acc, alpha, beta);
+}
+
 template <typename Z, typename V>
 int launch_combine(const fs_slices* sl, const void* z, void* y, double alpha, double beta,
                    hipStream_t st) {
   const int64_t nb = (sl->n_rows + kCombineRows - 1) / kCombineRows;
   if (nb == 0) return FS_OK;
-  sliced_combine_kernel<Z, V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
-      static_cast<const Z*>(z), sl->compact_row, sl->bounds, sl->n_slices, sl->n_rows, nb,
-      static_cast<V*>(y), alpha, beta);
+  if (sl->row_mask)
+    sliced_combine_mask_kernel<Z, V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
+        static_cast<const Z*>(z), sl->row_mask, sl->bounds, sl->n_slices, sl->n_rows, nb,
+        sl->n_compact, static_cast<V*>(y), alpha, beta);
+  else
+    sliced_combine_kernel<Z, V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
+        static_cast<const Z*>(z), sl->compact_row, sl->bounds, sl->n_slices, sl->n_rows, nb,
+        static_cast<V*>(y), alpha, beta);
   return check_launch("fs_sliced_combine");
 }
 
@@ -91,7 +158,7 @@ extern "C" int fs_sliced_combine(const fs_slices* sl, const void* z, void* y, do
   if (sl->rows_per_block != kCombineRows)
     return fail(FS_ERR_VALUE, "fs_sliced_combine: rows_per_block must be %d", kCombineRows);
   if (sl->n_rows == 0) return FS_OK;
-  if (!y || !sl->bounds || (sl->n_compact > 0 && (!z || !sl->compact_row)))
+  if (!y || !sl->bounds || (sl->n_compact > 0 && (!z || (!sl->compact_row && !sl->row_mask))))
     return fail(FS_ERR_VALUE, "fs_sliced_combine: null pointer");
   hipStream_t st = as_stream(stream);
   const bool sums = sl->z_type != sl->value_type;

@@ -39,6 +39,22 @@ def slice_count(n_cols: int, slice_cols: int) -> int:
     return N_GROUPS * q
 
 
+def row_masks(ukeys: torch.Tensor, n: int, n_slices: int, n_blocks: int) -> torch.Tensor:
+    """Presence bits of the combine: word (k * n_blocks + b) * 8 + j, bit i
+    set when row 256 b + 32 j + i has a compact row in slice k.  ``ukeys`` =
+    k * n + row of every compact row (distinct), as int32 words."""
+    words = COMBINE_ROWS // 32
+    acc = torch.zeros(n_slices * n_blocks * words, dtype=torch.int64, device=ukeys.device)
+    if ukeys.numel():
+        k = ukeys // max(n, 1)
+        r = ukeys % max(n, 1)
+        idx = (k * n_blocks + r // COMBINE_ROWS) * words + (r % COMBINE_ROWS) // 32
+        acc.index_add_(0, idx, torch.bitwise_left_shift(torch.ones_like(r), r % 32))
+    # distinct bits per word, so the sum is the OR; store as two's-complement int32
+    acc = torch.where(acc >= 2**31, acc - 2**32, acc)
+    return acc.to(torch.int32)
+
+
 class SlicedCsr:
     """Slice-major copy of one CSR matrix plus its compact-row plan."""
 
@@ -90,6 +106,7 @@ class SlicedCsr:
              + torch.arange(n_blocks + 1, device=dev, dtype=torch.int64)[None, :] * COMBINE_ROWS)
         q = torch.minimum(q, (torch.arange(P, device=dev, dtype=torch.int64)[:, None] + 1) * n)
         self.bounds = torch.searchsorted(ukeys, q.reshape(-1).contiguous(), side="left")
+        self.row_mask = row_masks(ukeys, n, P, n_blocks)
         del ukeys
         # entry tiles over the compact rows, dealt to XCD groups by slice
         T = schedule.tile_nnz if (schedule.tile_nnz > 0 and schedule.tile_mode == 1) else 2048
@@ -123,7 +140,8 @@ class SlicedCsr:
                                   n_rows=n, n_compact=self.n_compact,
                                   compact_row=_lib.ptr(self.compact_row),
                                   bounds=_lib.ptr(self.bounds), rows_per_block=COMBINE_ROWS,
-                                  z_type=_lib.value_code(z_dtype))
+                                  z_type=_lib.value_code(z_dtype),
+                                  row_mask=_lib.ptr(self.row_mask))
 
     def run(self, x, y, alpha, beta, device):
         """y = alpha A x + beta y: compact-row partials, then the combine."""

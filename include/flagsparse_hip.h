@@ -226,6 +226,14 @@ typedef struct fs_slices {
   int32_t rows_per_block;    /* 256                                        */
   int32_t z_type;            /* of the partials: value_type, or FS_F64 /
                                 FS_C128 for row sums (fs_plan.row_sums)     */
+  const uint32_t *row_mask;  /* null, or [n_slices * n_blocks * 8] words
+                                (rows_per_block 256): bit r % 32 of word
+                                (k * n_blocks + b) * 8 + (r % 256) / 32 is
+                                set when row r = 256 b + (r % 256) has a
+                                compact row in slice k.  With it, a row's
+                                partial in slice k is found by counting the
+                                set bits before it, and compact_row is not
+                                read.                                       */
 } fs_slices;
 
 int fs_sliced_combine(const fs_slices *sl, const void *z, void *y, double alpha, double beta,

@@ -216,6 +216,41 @@ def test_spmv_entry_tiles_all_dtypes(orc, dtype, index_dtype):
             assert (np.abs(got - want) / np.maximum(scale_, 1e-300)).max() <= bound
 
 
+@pytest.mark.parametrize("use_opt", [False, True], ids=["sums", "native"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.complex64],
+                         ids=["f32", "f64", "c64"])
+def test_sliced_combine_row_masks_equal_row_ids(dtype, use_opt):
+    """The combine's two ways to find a row's partials -- presence masks with
+    popcounts, or the compact-row ids -- give the same bits (fp64 row sums of
+    the default fp32 path, native partials otherwise), including 24 slices
+    (three groups of 8) and alpha / beta."""
+    from flagsparse_public_amd import _lib
+    npd = {torch.float32: np.float32, torch.float64: np.float64,
+           torch.complex64: np.complex64}[dtype]
+    ip, cols, vals = _powerlaw(30000, 5, 20000, np.float64)
+    n = ip.size - 1
+    rng = np.random.default_rng(8)
+    if dtype.is_complex:
+        vals = vals + 1j * rng.uniform(-1, 1, vals.size)
+    vals = vals.astype(npd)
+    x = rng.uniform(-1, 1, n).astype(npd)
+    for slice_cols in (4000, 1250):
+        sch = Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=slice_cols)
+        p = prepare_spmv_csr(to_dev(vals), to_dev(cols), to_dev(ip), (n, n), schedule=sch)
+        flagsparse_spmv_csr(x=to_dev(x), prepared=p, use_opt=use_opt)  # fills the partials
+        sliced = [v for k, v in p._plans.items() if isinstance(k, tuple) and k[0] == "sliced"]
+        assert sliced
+        s = sliced[-1]
+        assert s.n_slices == (8 if slice_cols == 4000 else 24)
+        no_mask = _lib.FsSlices(**{f: getattr(s.desc, f) for f, _ in _lib.FsSlices._fields_})
+        no_mask.row_mask = None
+        y0 = to_dev(rng.uniform(-1, 1, n).astype(npd))
+        y1, y2 = y0.clone(), y0.clone()
+        _lib.sliced_combine(s.desc, s.z, y1, 1.5, -0.25, y1.device)
+        _lib.sliced_combine(no_mask, s.z, y2, 1.5, -0.25, y2.device)
+        assert torch.equal(y1, y2), slice_cols
+
+
 def test_spmv_long_rows_compensated_fp32(orc):
     """Rows far above 4096 entries with heavy cancellation stay within 1e-5 of
     sum|a x| on the native fp32 path (compensated chunk accumulation)."""

@@ -328,6 +328,19 @@ def test_sliced_layout_structure():
     gt = s.group_tiles.tolist()
     assert gt[0] == 0 and gt[-1] == s.plan.n_tiles and gt == sorted(gt)
     assert s.group_blocks == max(np.diff(gt))
+    # row masks: bit set exactly for the (slice, row) pairs with a compact
+    # row, and the popcount rule finds each compact row's index
+    nb = b.shape[1] - 1
+    words = s.row_mask.numpy().view(np.uint32).reshape(s.n_slices, nb, 8)
+    bits = ((words[..., None] >> np.arange(32, dtype=np.uint32)) & 1).reshape(s.n_slices, nb * 256)
+    present = np.zeros((s.n_slices, nb * 256), np.uint8)
+    present[sl, cr] = 1
+    assert np.array_equal(bits, present)
+    for i in np.random.default_rng(1).choice(s.n_compact, 2000, replace=False):
+        k, r = sl[i], cr[i]
+        bb = r // slicing.COMBINE_ROWS
+        rank = int(bits[k, bb * 256:r].sum())
+        assert b[k, bb] + rank == i
 
 
 def test_synthetic_generators_deterministic_and_sorted():
