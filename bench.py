@@ -78,10 +78,61 @@ def cpu_baseline(ip, cols, vals, x, nbytes, budget_s=10.0):
         t_total += time.perf_counter() - t0
         reps += 1
     sec = t_total / reps
-    return ({"value": round(nbytes / sec / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-             "sample": f"{reps} whole-matrix passes of the C oracle (fp64 accumulate, "
-                       f"sequential, 1 thread), {sec * 1e3:.1f} ms each; GB/s over the same "
-                       f"fp32 algorithmic bytes"}, y64)
+    res = {"value": round(nbytes / sec / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+           "sample": f"{reps} whole-matrix passes of the C oracle (fp64 accumulate, "
+                     f"sequential, 1 thread), {sec * 1e3:.1f} ms each; GB/s over the same "
+                     f"fp32 algorithmic bytes",
+           "host_cpus": os.cpu_count(), "library_cpu_paths": library_cpu_paths(ip, cols, vals, x,
+                                                                               nbytes)}
+    return res, y64
+
+
+def library_cpu_paths(ip, cols, vals, x, nbytes, reps=5):
+    """The CPU library routines a user would otherwise call on the same fp32
+    matrix: SciPy's CSR matvec (serial) and torch's CPU CSR product (all
+    host threads); median of `reps` after one warm-up, GB/s over the same
+    algorithmic bytes.  Informational, beside the oracle's figure."""
+    import statistics
+    import numpy as np
+    out = []
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    n = ip.size - 1
+    try:
+        import scipy.sparse as sp
+        A = sp.csr_matrix((vals, cols, ip), shape=(n, n))
+        sec = timed(lambda: A @ x)
+        out.append({"library": "scipy csr_matrix @ x (fp32, serial)", "ms": round(sec * 1e3, 2),
+                    "GB/s": round(nbytes / sec / 1e9, 3), "threads": 1})
+        del A
+    except Exception as exc:  # reported, not fatal: a baseline only
+        out.append({"library": "scipy", "error": str(exc)})
+    try:
+        import torch
+        threads = os.cpu_count() or 1
+        torch.set_num_threads(threads)
+        import warnings
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", UserWarning)  # "sparse CSR support is in beta"
+            A = torch.sparse_csr_tensor(torch.from_numpy(np.asarray(ip, np.int64)),
+                                        torch.from_numpy(np.asarray(cols, np.int64)),
+                                        torch.from_numpy(vals), size=(n, n))
+        xt = torch.from_numpy(x)
+        sec = timed(lambda: torch.mv(A, xt))
+        out.append({"library": "torch CPU CSR mv (fp32)", "ms": round(sec * 1e3, 2),
+                    "GB/s": round(nbytes / sec / 1e9, 3), "threads": torch.get_num_threads()})
+        del A
+    except Exception as exc:
+        out.append({"library": "torch CPU", "error": str(exc)})
+    return out
 
 
 def measure_modes(d_vals, d_cols, d_ip, d_x, n, nnz, prepared, use_opt, steps):

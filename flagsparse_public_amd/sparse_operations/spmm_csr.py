@@ -502,6 +502,117 @@ def _hip_spmm_csr_impl_opt_prepared(prepared, B, out=None, alpha=1.0, beta=0.0, 
 _triton_spmm_csr_impl_opt_prepared = _hip_spmm_csr_impl_opt_prepared
 
 
+# ------------------------------------------------------- diagnose candidate
+#
+# The reference's diagnose script (tests/diagnose_spmm_opt.py:311-330, 527)
+# labels rows by the candidate bucket table of spmm_csr.py:1457-1497 and
+# compares a candidate SpMM against the opt path.  The table is kept as
+# returned metadata; the candidate values come from the HIP SpMM.
+
+_SPMM_OPT_CANDIDATE_BUCKET_SPECS_F32 = (
+    {"label": "short_16", "kind": "micro", "min_row_nnz": 0, "max_row_nnz": 16, "block_nnz": 16},
+    {"label": "short_32", "kind": "micro", "min_row_nnz": 17, "max_row_nnz": 32, "block_nnz": 32},
+    {"label": "short_64", "kind": "segmented", "min_row_nnz": 33, "max_row_nnz": 64,
+     "block_nnz": 64, "segments": 2},
+    {"label": "short_128", "kind": "segmented", "min_row_nnz": 65, "max_row_nnz": 128,
+     "block_nnz": 128, "segments": 4},
+    {"label": "vector_192", "kind": "segmented", "min_row_nnz": 129, "max_row_nnz": 192,
+     "block_nnz": 64, "segments": 2},
+    {"label": "vector_256", "kind": "segmented", "min_row_nnz": 193, "max_row_nnz": 256,
+     "block_nnz": 64, "segments": 4},
+    {"label": "vector_512", "kind": "segmented", "min_row_nnz": 257, "max_row_nnz": 512,
+     "block_nnz": 64, "segments": 4},
+    {"label": "vector_1024", "kind": "segmented", "min_row_nnz": 513, "max_row_nnz": 1024,
+     "block_nnz": 32, "segments": 8},
+    {"label": "vector_2048", "kind": "segmented", "min_row_nnz": 1025, "max_row_nnz": 2048,
+     "block_nnz": 32, "segments": 8},
+)
+_SPMM_OPT_CANDIDATE_BUCKET_SPECS_F64 = (
+    {"label": "short_16", "kind": "segmented", "min_row_nnz": 0, "max_row_nnz": 16,
+     "block_nnz": 16, "segments": 2},
+    {"label": "short_32", "kind": "segmented", "min_row_nnz": 17, "max_row_nnz": 32,
+     "block_nnz": 32, "segments": 2},
+    {"label": "short_64", "kind": "segmented", "min_row_nnz": 33, "max_row_nnz": 64,
+     "block_nnz": 64, "segments": 2},
+    {"label": "short_128", "kind": "segmented", "min_row_nnz": 65, "max_row_nnz": 128,
+     "block_nnz": 128, "segments": 4},
+    {"label": "vector_256", "kind": "segmented", "min_row_nnz": 129, "max_row_nnz": 256,
+     "block_nnz": 64, "segments": 4},
+    {"label": "vector_512", "kind": "segmented", "min_row_nnz": 257, "max_row_nnz": 512,
+     "block_nnz": 64, "segments": 4},
+    {"label": "vector_1024", "kind": "segmented", "min_row_nnz": 513, "max_row_nnz": 1024,
+     "block_nnz": 32, "segments": 8},
+    {"label": "vector_2048", "kind": "segmented", "min_row_nnz": 1025, "max_row_nnz": 2048,
+     "block_nnz": 32, "segments": 8},
+)
+_SPMM_OPT_CANDIDATE_SPLIT_THRESHOLD_F32 = 2048
+_SPMM_OPT_CANDIDATE_SPLIT_THRESHOLD_F64 = 2048
+
+
+def _select_spmm_opt_candidate_specs(dtype):
+    return (_SPMM_OPT_CANDIDATE_BUCKET_SPECS_F64 if dtype == torch.float64
+            else _SPMM_OPT_CANDIDATE_BUCKET_SPECS_F32)
+
+
+def _select_spmm_opt_candidate_split_threshold(dtype):
+    return (_SPMM_OPT_CANDIDATE_SPLIT_THRESHOLD_F64 if dtype == torch.float64
+            else _SPMM_OPT_CANDIDATE_SPLIT_THRESHOLD_F32)
+
+
+def _build_spmm_opt_candidate_buckets(prepared):
+    """Rows of each candidate bucket (spmm_csr.py:1835-1873): a length range
+    per spec (the first takes every row up to its maximum, empty rows
+    included), then a "split" bucket of the rows over the threshold.  Row
+    ids are int32 unless the row count needs int64."""
+    row_lengths = prepared.row_lengths
+    device = row_lengths.device
+    row_count = int(row_lengths.numel())
+    index_dtype = torch.int32 if row_count - 1 <= _INDEX_LIMIT_INT32 else torch.int64
+    all_rows = torch.arange(row_count, device=device, dtype=index_dtype)
+    buckets = []
+    for spec in _select_spmm_opt_candidate_specs(prepared.data.dtype):
+        lo, hi = int(spec["min_row_nnz"]), int(spec["max_row_nnz"])
+        mask = row_lengths <= hi if lo <= 0 else (row_lengths >= lo) & (row_lengths <= hi)
+        rows = all_rows[mask]
+        if rows.numel() == 0:
+            continue
+        bucket = {"label": spec["label"], "kind": spec["kind"], "rows": rows,
+                  "block_nnz": int(spec["block_nnz"]), "min_row_nnz": lo, "max_row_nnz": hi}
+        if "segments" in spec:
+            bucket["segments"] = int(spec["segments"])
+        buckets.append(bucket)
+    split_rows = all_rows[row_lengths > _select_spmm_opt_candidate_split_threshold(
+        prepared.data.dtype)]
+    if split_rows.numel() > 0:
+        buckets.append({"label": "split", "kind": "split", "rows": split_rows, "block_nnz": 256})
+    return buckets
+
+
+def _hip_spmm_csr_impl_opt_prepared_candidate(prepared, B):
+    """The diagnose candidate (spmm_csr.py:1995-2026): opt-path validation,
+    then C with every row accumulated in fp64 and rounded once (the
+    reference's candidate compensates short rows with Kahan sums; fp64
+    accumulation is at least as accurate).  Returns (C, False): no long-row
+    fallback exists here."""
+    _check_opt_B(prepared, B)
+    B = B.contiguous()
+    n_dense = int(B.shape[1])
+    sch, col_tile = spmm_launch_schedule(prepared, n_dense)
+    plan = prepared.plan(_lib.FS_ACC_WIDE, n_dense, sch)
+    C = _launch_spmm(prepared.csr_struct(), plan, B, prepared.n_rows, n_dense, B.dtype, B.device,
+                     None, 1.0, 0.0, col_tile)
+    return C, False
+
+
+_triton_spmm_csr_impl_opt_prepared_candidate = _hip_spmm_csr_impl_opt_prepared_candidate
+
+
+def _flagsparse_spmm_csr_opt_candidate_for_diagnose(prepared, B):
+    """C of the diagnose candidate (spmm_csr.py:2029-2031)."""
+    C_out, _ = _hip_spmm_csr_impl_opt_prepared_candidate(prepared, B)
+    return C_out
+
+
 def flagsparse_spmm_csr_opt(data=None, indices=None, indptr=None, B=None, shape=None,
                             prepared=None, out=None, return_time=False, alpha=1.0, beta=0.0):
     """Native fp32 / fp64 SpMM-opt (spmm_csr.py:2160-2214)."""

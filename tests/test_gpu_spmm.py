@@ -222,3 +222,27 @@ def test_spmm_banded_1m_workload(orc, n_dense, dtype):
     S = orc.spmm_abs(ip, cols, vals, B)
     p = prepare_spmm_csr_opt(to_dev(vals), to_dev(cols), to_dev(ip), (n, n))
     _check(flagsparse_spmm_csr_opt(B=to_dev(B), prepared=p), C64, S, dtype, "opt")
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+def test_spmm_diagnose_candidate(orc, dtype):
+    """The diagnose hooks (spmm_csr.py:1835-2031): candidate rows labelled by
+    bucket, and the candidate C accumulated in fp64 -- for fp32 within one
+    ulp of the rounded fp64 oracle and bit-identical on nearly every entry."""
+    from flagsparse_public_amd.sparse_operations import spmm_csr as m
+    rng = np.random.default_rng(12)
+    npd = NP[dtype]
+    lens = np.concatenate([rng.poisson(20, 600), [0, 3000, 2100, 700, 129]])
+    ip = np.zeros(lens.size + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    m_cols = 4000
+    cols = rng.integers(0, m_cols, ip[-1]).astype(np.int32)
+    vals = rng.uniform(-1, 1, ip[-1]).astype(npd)
+    B = rng.uniform(-1, 1, (m_cols, 24)).astype(npd)
+    C64 = orc.spmm(ip, cols, vals, B)
+    S = orc.spmm_abs(ip, cols, vals, B)
+    p = prepare_spmm_csr_opt(to_dev(vals), to_dev(cols), to_dev(ip), (lens.size, m_cols))
+    labels = {b["label"] for b in m._build_spmm_opt_candidate_buckets(p)}
+    assert "split" in labels and "short_16" in labels
+    C = m._flagsparse_spmm_csr_opt_candidate_for_diagnose(p, to_dev(B))
+    _check(C, C64, S, dtype, "default")

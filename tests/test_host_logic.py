@@ -421,3 +421,37 @@ def test_batched_cuts_small_matrices():
     ip = torch.tensor([0, 2, 3, 3])
     assert _cuts(ip, 12.0, 96 * 1024, 2048, 4096).tolist() == [0, 3]
     assert _cuts(torch.tensor([0]), 12.0, 100, 4).tolist() == [0]
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_spmm_candidate_buckets_match_length_ranges(dtype):
+    """The diagnose candidate buckets (spmm_csr.py:1835-1873): every row with
+    at most 2048 entries lands in the one spec range holding its length (the
+    first range also takes empty rows), rows over 2048 go to "split", labels
+    and kinds follow the dtype's table."""
+    from types import SimpleNamespace
+    from flagsparse_public_amd.sparse_operations import spmm_csr as m
+    rng = np.random.default_rng(4)
+    lens = np.concatenate([rng.integers(0, 2200, 3000), [0, 16, 17, 32, 33, 2048, 2049, 5000]])
+    prepared = SimpleNamespace(row_lengths=torch.from_numpy(lens.astype(np.int64)),
+                               data=torch.empty(0, dtype=dtype))
+    buckets = m._build_spmm_opt_candidate_buckets(prepared)
+    specs = m._select_spmm_opt_candidate_specs(dtype)
+    seen = np.zeros(lens.size, np.int64)
+    for b in buckets:
+        rows = b["rows"].numpy()
+        assert b["rows"].dtype == torch.int32
+        seen[rows] += 1
+        if b["label"] == "split":
+            assert b["kind"] == "split" and np.all(lens[rows] > 2048)
+            assert set(rows) == set(np.nonzero(lens > 2048)[0])
+            continue
+        spec = next(sp for sp in specs if sp["label"] == b["label"])
+        lo, hi = spec["min_row_nnz"], spec["max_row_nnz"]
+        want = np.nonzero((lens <= hi) & ((lens >= lo) if lo > 0 else True))[0]
+        assert np.array_equal(np.sort(rows), want)
+        assert b["kind"] == spec["kind"] and b["block_nnz"] == spec["block_nnz"]
+        assert b.get("segments") == spec.get("segments")
+    assert np.all(seen == 1)
+    kinds = {sp["label"]: sp["kind"] for sp in specs}
+    assert kinds["short_16"] == ("segmented" if dtype == torch.float64 else "micro")
