@@ -1,4 +1,4 @@
-// vendor_rocsparse.cpp -- rocSPARSE CSR SpMV / SpMM behind a tiny C ABI, built
+// vendor_rocsparse.cpp -- rocSPARSE CSR SpMV / SpMM / SDDMM behind a tiny C ABI, built
 // as its own library (libfs_vendor.so).  It is the GPU vendor baseline the
 // benchmark harness times beside this library's kernels, in the slot where the
 // reference's harness times cuSPARSE through CuPy (benchmarks.py:345-507,
@@ -32,14 +32,16 @@ int set_err(const char* what, int code) {
     if (st_ != rocsparse_status_success) return set_err(#call, (int)st_); \
   } while (0)
 
+enum Kind { kSpmv, kSpmm, kSddmm };
+
 struct Ctx {
   rocsparse_handle handle = nullptr;
   rocsparse_spmat_descr mat = nullptr;
   rocsparse_dnvec_descr x = nullptr, y = nullptr;
-  rocsparse_dnmat_descr B = nullptr, C = nullptr;
+  rocsparse_dnmat_descr B = nullptr, C = nullptr;  // SDDMM: the dense X and Y^T
   rocsparse_datatype compute = rocsparse_datatype_f32_r;
   int alg = 0;
-  bool spmm = false;
+  Kind kind = kSpmv;
   void* buffer = nullptr;
   size_t buffer_bytes = 0;
 };
@@ -119,6 +121,26 @@ int spmm_stage(Ctx* c, rocsparse_spmm_stage stage, const Scalars& s) {
   return 0;
 }
 
+// SDDMM stages: the buffer query, the preprocess and the compute call share
+// their arguments.
+int sddmm_stage(Ctx* c, int stage, const Scalars& s) {
+  const rocsparse_operation N = rocsparse_operation_none;
+  const rocsparse_sddmm_alg alg = (rocsparse_sddmm_alg)c->alg;
+  if (stage == 0) {
+    size_t bytes = 0;
+    RS(rocsparse_sddmm_buffer_size(c->handle, N, N, s.a, c->B, c->C, s.b, c->mat, c->compute, alg,
+                                   &bytes));
+    c->buffer_bytes = bytes;
+  } else if (stage == 1) {
+    RS(rocsparse_sddmm_preprocess(c->handle, N, N, s.a, c->B, c->C, s.b, c->mat, c->compute, alg,
+                                  c->buffer));
+  } else {
+    RS(rocsparse_sddmm(c->handle, N, N, s.a, c->B, c->C, s.b, c->mat, c->compute, alg,
+                       c->buffer));
+  }
+  return 0;
+}
+
 int alloc_buffer(Ctx* c) {
   if (c->buffer_bytes == 0) return 0;
   if (hipMalloc(&c->buffer, c->buffer_bytes) != hipSuccess) return set_err("hipMalloc", -1);
@@ -148,7 +170,7 @@ int spmm_setup(Ctx* c, const fs_csr* A, const void* B, int64_t n_dense, void* C,
   if (!map_value(A->value_type, &vt)) return set_err("value type", -2);
   c->compute = vt;
   c->alg = alg;
-  c->spmm = true;
+  c->kind = kSpmm;
   RS(rocsparse_create_handle(&c->handle));
   RS(rocsparse_set_stream(c->handle, (hipStream_t)stream));
   int rc = make_matrix(c, A, vt);
@@ -160,6 +182,30 @@ int spmm_setup(Ctx* c, const fs_csr* A, const void* B, int64_t n_dense, void* C,
   if ((rc = spmm_stage(c, rocsparse_spmm_stage_buffer_size, s))) return rc;
   if ((rc = alloc_buffer(c))) return rc;
   return spmm_stage(c, rocsparse_spmm_stage_preprocess, s);
+}
+
+// out[p] = alpha * <x[row p], y[col p]> + beta * out[p]: X = x [n_rows, k]
+// row-major, Y^T = y read as a [k, n_cols] column-major matrix, the sampled
+// product written into the pattern's values (which must point at out).
+int sddmm_setup(Ctx* c, const fs_csr* P, const void* x, const void* y, int64_t k, int alg,
+                void* stream) {
+  rocsparse_datatype vt;
+  if (!map_value(P->value_type, &vt)) return set_err("value type", -2);
+  c->compute = vt;
+  c->alg = alg;
+  c->kind = kSddmm;
+  RS(rocsparse_create_handle(&c->handle));
+  RS(rocsparse_set_stream(c->handle, (hipStream_t)stream));
+  int rc = make_matrix(c, P, vt);
+  if (rc) return rc;
+  RS(rocsparse_create_dnmat_descr(&c->B, P->n_rows, k, k, const_cast<void*>(x), vt,
+                                  rocsparse_order_row));
+  RS(rocsparse_create_dnmat_descr(&c->C, k, P->n_cols, k, const_cast<void*>(y), vt,
+                                  rocsparse_order_column));
+  const Scalars s = scalars(vt, 1.0, 0.0);
+  if ((rc = sddmm_stage(c, 0, s))) return rc;
+  if ((rc = alloc_buffer(c))) return rc;
+  return sddmm_stage(c, 1, s);
 }
 
 }  // namespace
@@ -191,13 +237,31 @@ int fsv_spmm_setup(const fs_csr* A, const void* B, int64_t n_dense, void* C, int
   return 0;
 }
 
-// Enqueues the compute stage (y = alpha A x + beta y, or the SpMM analogue).
+// Binds the sampled product of the pattern P (P->values = the output) with
+// x [n_rows, k] and y [n_cols, k], both row-major; `alg` is a
+// rocsparse_sddmm_alg value.
+int fsv_sddmm_setup(const fs_csr* P, const void* x, const void* y, int64_t k, int alg,
+                    void* stream, void** ctx) {
+  if (!P || !ctx) return set_err("null argument", -3);
+  Ctx* c = new (std::nothrow) Ctx;
+  if (!c) return set_err("new", -4);
+  const int rc = sddmm_setup(c, P, x, y, k, alg, stream);
+  if (rc) { destroy(c); *ctx = nullptr; return rc; }
+  *ctx = c;
+  return 0;
+}
+
+// Enqueues the compute stage (y = alpha A x + beta y, or the SpMM / SDDMM
+// analogue).
 int fsv_run(void* ctx, double alpha, double beta) {
   Ctx* c = static_cast<Ctx*>(ctx);
   if (!c) return set_err("null context", -3);
   const Scalars s = scalars(c->compute, alpha, beta);
-  return c->spmm ? spmm_stage(c, rocsparse_spmm_stage_compute, s)
-                 : spmv_stage(c, rocsparse_spmv_stage_compute, s);
+  switch (c->kind) {
+    case kSpmm: return spmm_stage(c, rocsparse_spmm_stage_compute, s);
+    case kSddmm: return sddmm_stage(c, 2, s);
+    default: return spmv_stage(c, rocsparse_spmv_stage_compute, s);
+  }
 }
 
 void fsv_destroy(void* ctx) { destroy(static_cast<Ctx*>(ctx)); }

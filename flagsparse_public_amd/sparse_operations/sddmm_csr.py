@@ -264,9 +264,8 @@ def flagsparse_sddmm_csr(data=None, indices=None, indptr=None, x=None, y=None, s
 
 def benchmark_sddmm_case(n_rows=1024, n_cols=1024, nnz=16384, k_dim=64, value_dtype=torch.float32,
                          warmup=10, iters=30, alpha=1.0, beta=0.0, run_cusparse=False):
-    """SDDMM timed beside the torch sampled-dot computation; result keys of
-    sddmm_csr.py:472-571.  The reference's vendor column is a CuPy sampled dot,
-    which has no ROCm build here, so it reports the reason instead."""
+    """SDDMM timed beside the torch sampled-dot computation and, with
+    run_cusparse, rocSPARSE's SDDMM; result keys of sddmm_csr.py:472-571."""
     if value_dtype not in SUPPORTED_SDDMM_VALUE_DTYPES:
         raise TypeError("value_dtype must be torch.float32 or torch.float64")
     device = torch.device("cuda")
@@ -286,18 +285,45 @@ def benchmark_sddmm_case(n_rows=1024, n_cols=1024, nnz=16384, k_dim=64, value_dt
     atol, rtol = _tolerance_for_dtype(value_dtype)
     match = bool(torch.allclose(values, ref_values, atol=atol, rtol=rtol))
     max_abs = float(torch.max(torch.abs(values - ref_values)).item()) if values.numel() > 0 else 0.0
-    reason = "CuPy sampled-dot reference is not available on ROCm" if run_cusparse else None
+    vendor_ms = vendor_match = vendor_values = None
+    reason = None
+    if run_cusparse:
+        # the vendor column: rocSPARSE's SDDMM in the slot of the reference's
+        # CuPy sampled dot (sddmm_csr.py:515-538)
+        from .. import vendor
+        if not vendor.available():
+            reason = "rocSPARSE baseline unavailable"
+        elif prepared.nnz == 0:
+            reason = "empty pattern"
+        else:
+            try:
+                vout = torch.empty(prepared.nnz, dtype=value_dtype, device=device)
+                bound = vendor.VendorSddmm(prepared.indices, prepared.indptr, prepared.shape,
+                                           x.contiguous(), y.contiguous(), vout)
+
+                def vrun():
+                    if beta != 0.0:
+                        vout.copy_(data)
+                    bound.run(alpha, beta)
+                    return vout
+                vendor_values, vendor_ms = _benchmark_cuda_op(vrun, warmup=warmup, iters=iters)
+                vendor_match = bool(torch.allclose(vendor_values, ref_values, atol=atol,
+                                                   rtol=rtol))
+            except Exception as exc:  # reported in backend_status like the reference
+                reason = str(exc)
     return {
         "parameters": {"n_rows": n_rows, "n_cols": n_cols, "nnz": nnz, "k_dim": k_dim,
                        "value_dtype": str(value_dtype), "warmup": warmup, "iters": iters,
                        "alpha": alpha, "beta": beta},
-        "performance": {"triton_ms": ms, "pytorch_ms": pytorch_ms, "cusparse_ms": None,
+        "performance": {"triton_ms": ms, "pytorch_ms": pytorch_ms, "cusparse_ms": vendor_ms,
                         "triton_speedup_vs_pytorch": (pytorch_ms / ms if ms > 0 else None),
-                        "triton_speedup_vs_cusparse": None},
+                        "triton_speedup_vs_cusparse": (vendor_ms / ms if (vendor_ms and ms > 0)
+                                                       else None)},
         "verification": {"triton_match_pytorch": match, "triton_max_abs_error": max_abs,
-                         "cusparse_match_pytorch": None},
-        "backend_status": {"cusparse_unavailable_reason": reason},
-        "samples": {"triton": values, "pytorch": ref_values},
+                         "cusparse_match_pytorch": vendor_match},
+        "backend_status": {"cusparse_unavailable_reason": reason,
+                           "vendor_library": "rocSPARSE"},
+        "samples": {"triton": values, "pytorch": ref_values, "cusparse": vendor_values},
     }
 
 

@@ -45,11 +45,13 @@ def load():
                                    ctypes.POINTER(vp)]
     lib.fsv_spmm_setup.argtypes = [ctypes.POINTER(_lib.FsCsr), vp, ctypes.c_int64, vp,
                                    ctypes.c_int, vp, ctypes.POINTER(vp)]
+    lib.fsv_sddmm_setup.argtypes = [ctypes.POINTER(_lib.FsCsr), vp, vp, ctypes.c_int64,
+                                    ctypes.c_int, vp, ctypes.POINTER(vp)]
     lib.fsv_run.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     lib.fsv_destroy.argtypes = [vp]
     lib.fsv_destroy.restype = None
     lib.fsv_last_error.restype = ctypes.c_char_p
-    for f in ("fsv_spmv_setup", "fsv_spmm_setup", "fsv_run"):
+    for f in ("fsv_spmv_setup", "fsv_spmm_setup", "fsv_sddmm_setup", "fsv_run"):
         getattr(lib, f).restype = ctypes.c_int
     _lib_handle = lib
     return lib
@@ -115,6 +117,28 @@ class VendorSpmm(_Bound):
         _check(self._lib, rc, f"spmm setup ({alg})")
         self._ctx = ctx
         self._keep = (data, indices, indptr, B, C, csr)
+
+
+class VendorSddmm(_Bound):
+    """out[p] = alpha <x[row p], y[col p]> + beta out[p] with rocSPARSE's
+    SDDMM over the CSR pattern (indices, indptr); x [n_rows, k] and y
+    [n_cols, k] row-major; `out` holds the pattern's values."""
+
+    def __init__(self, indices, indptr, shape, x, y, out):
+        super().__init__()
+        self._lib = load()
+        if self._lib is None:
+            raise RuntimeError(f"rocSPARSE baseline unavailable: {_load_error}")
+        if not (x.is_contiguous() and y.is_contiguous() and out.is_contiguous()):
+            raise ValueError("x, y and out must be contiguous")
+        csr = _lib.make_csr(out, indices, indptr, int(shape[0]), int(shape[1]))
+        ctx = ctypes.c_void_p()
+        rc = self._lib.fsv_sddmm_setup(ctypes.byref(csr), x.data_ptr(), y.data_ptr(),
+                                       int(x.shape[1]), 0, _lib.stream_handle(x.device),
+                                       ctypes.byref(ctx))
+        _check(self._lib, rc, "sddmm setup")
+        self._ctx = ctx
+        self._keep = (indices, indptr, x, y, out, csr)
 
 
 def available() -> bool:
