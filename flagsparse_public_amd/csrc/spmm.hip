@@ -152,6 +152,64 @@ __device__ __forceinline__ void mm_accumulate(A (&acc)[VW], const I* __restrict_
   }
 }
 
+// A short row's entries, loaded G at a time by the G lanes of its group (lane
+// g loads entry j0 + g: one coalesced col / value load per G entries instead
+// of one broadcast load per entry, which cuts the L2 requests of the entry
+// stream), parked in the group's LDS slots and read back U at a time.  Lanes
+// past the last dense column (colok false) still load their share.  The
+// products are added in entry order, as in mm_accumulate.
+template <int VW, int U, int G, typename A, typename V>
+__device__ __forceinline__ void mm_accumulate_group(A (&acc)[VW], int32_t* slot_col,
+                                                    typename A::compute_t* slot_val,
+                                                    const int32_t* __restrict__ idx,
+                                                    const V* __restrict__ val,
+                                                    const V* __restrict__ B, int64_t ldb,
+                                                    int64_t n0, bool colok, int64_t s,
+                                                    int64_t e, int g) {
+  using C = typename A::compute_t;
+  static_assert(G % U == 0, "batches of U entries tile the group's G slots");
+  for (int64_t j0 = s; j0 < e; j0 += G) {
+    const int64_t jj = j0 + g;
+    slot_col[g] = jj < e ? idx[jj] : 0;
+    slot_val[g] = jj < e ? load_as<C>(val, jj) : zero<C>();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int cnt = e - j0 < G ? (int)(e - j0) : G;
+    if (colok) {
+      for (int u0 = 0; u0 < cnt; u0 += U) {
+        int64_t col[U];
+        C a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool ok = u0 + u < cnt;
+          col[u] = ok ? (int64_t)slot_col[u0 + u] : 0;
+          a[u] = ok ? slot_val[u0 + u] : zero<C>();
+        }
+        C b[U][VW];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (u0 + u < cnt) {
+            load_vec<C, VW>(b[u], B, col[u] * ldb + n0);
+          } else {
+#pragma unroll
+            for (int k = 0; k < VW; ++k) b[u][k] = zero<C>();
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (u0 + u < cnt)
+#pragma unroll
+            for (int k = 0; k < VW; ++k) acc[k].add_prod(a[u], b[u][k]);
+      }
+    }
+    // the slots are rewritten by the next batch
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+}
+
 template <typename V, typename A, typename I, typename P, int G, int VW, int U>
 __device__ __forceinline__ void spmm_chunk_role(const SpmmParams<V, I, P>& p, int64_t bid) {
   using CA = typename MmChunkAcc<A>::T;
@@ -209,6 +267,15 @@ __global__ __launch_bounds__(kMmBlock) void spmm_fixup_kernel(SpmmParams<V, I, P
   }
 }
 
+template <typename V, typename I>
+constexpr bool kHotSpmm = (std::is_same<V, float>::value || std::is_same<V, double>::value) &&
+                          std::is_same<I, int32_t>::value;
+
+// Short rows load their entries group-cooperatively for the fp32 / fp64,
+// int32-column kernels whose lane groups hold whole batches.
+template <typename V, typename I, int G, int U>
+constexpr bool kGroupLoads = kHotSpmm<V, I> && G >= U && G % U == 0;
+
 template <typename V, typename A, typename I, typename P, int G, int VW, int U>
 __global__ __launch_bounds__(kMmBlock) void spmm_csr_kernel(SpmmParams<V, I, P> p) {
   const int64_t bid = blockIdx.x;
@@ -220,23 +287,30 @@ __global__ __launch_bounds__(kMmBlock) void spmm_csr_kernel(SpmmParams<V, I, P> 
   if (p.xcd_remap) sb = xcd_remap_block(sb, (int64_t)gridDim.x - p.n_chunk_blocks);
   const int64_t tile = sb % p.n_tiles, rb = sb / p.n_tiles;
   const int64_t row = rb * (kMmBlock / G) + threadIdx.x / G;
-  if (row >= p.n_rows) return;
+  if (row >= p.n_rows) return;  // the whole group
   const int g = threadIdx.x % G;
   const int64_t n0 = tile * (G * VW) + (int64_t)g * VW;
-  if (n0 >= p.n_dense) return;  // lanes past the last column own nothing
   const int64_t rs = (int64_t)p.ptr[row], re = (int64_t)p.ptr[row + 1];
-  if (re - rs > p.short_max) return;  // chunked elsewhere
+  if (re - rs > p.short_max) return;  // chunked elsewhere (the whole group)
   A acc[VW];
 #pragma unroll
   for (int k = 0; k < VW; ++k) acc[k].init();
-  mm_accumulate<VW, U>(acc, p.idx, p.val, p.B, p.ldb, n0, rs, re, 1);
+  if constexpr (kGroupLoads<V, I, G, U>) {
+    using C = typename A::compute_t;
+    __shared__ int32_t slot_col[kMmBlock];
+    __shared__ C slot_val[kMmBlock];
+    const int first = threadIdx.x - g;  // the group's slots
+    const bool colok = n0 < p.n_dense;
+    mm_accumulate_group<VW, U, G>(acc, slot_col + first, slot_val + first, p.idx, p.val, p.B,
+                                  p.ldb, n0, colok, rs, re, g);
+    if (!colok) return;
+  } else {
+    if (n0 >= p.n_dense) return;  // lanes past the last column own nothing
+    mm_accumulate<VW, U>(acc, p.idx, p.val, p.B, p.ldb, n0, rs, re, 1);
+  }
 #pragma unroll
   for (int k = 0; k < VW; ++k) write_out(p.C, row * p.ldc + n0 + k, acc[k].value(), p.alpha, p.beta);
 }
-
-template <typename V, typename I>
-constexpr bool kHotSpmm = (std::is_same<V, float>::value || std::is_same<V, double>::value) &&
-                          std::is_same<I, int32_t>::value;
 
 template <typename V, typename A, typename I, typename P, int G, int VW>
 int launch_spmm_gv(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t ldc,
