@@ -126,11 +126,35 @@ __device__ void bat_spmm_rows(const BatchMatrix& m, int64_t r0, int64_t r1, doub
 #pragma unroll
           for (int k = 0; k < VW; ++k) acc[k] = __builtin_fmaf(a[u], b[u][k], acc[k]);
       }
-      for (; j < re; ++j) {
-        const int64_t c = m.col[j];
-        const float a = m.val[j];
+      if (j < re) {  // the last, partial batch: its loads in flight together
+        int64_t col[4];
+        float a[4];
 #pragma unroll
-        for (int k = 0; k < VW; ++k) acc[k] = __builtin_fmaf(a, m.B[c * m.ldb + n0 + k], acc[k]);
+        for (int u = 0; u < 4; ++u) {
+          const bool ok = j + u < re;
+          col[u] = ok ? m.col[j + u] : 0;
+          a[u] = ok ? m.val[j + u] : 0.f;
+        }
+        float b[4][VW];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+          for (int k = 0; k < VW; ++k) b[u][k] = 0.f;
+          if (j + u < re) {
+            if constexpr (VW == 4) {
+              const float4 v = *reinterpret_cast<const float4*>(m.B + col[u] * m.ldb + n0);
+              b[u][0] = v.x; b[u][1] = v.y; b[u][2] = v.z; b[u][3] = v.w;
+            } else {
+#pragma unroll
+              for (int k = 0; k < VW; ++k) b[u][k] = m.B[col[u] * m.ldb + n0 + k];
+            }
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (j + u < re)
+#pragma unroll
+            for (int k = 0; k < VW; ++k) acc[k] = __builtin_fmaf(a[u], b[u][k], acc[k]);
       }
 #pragma unroll
       for (int k = 0; k < VW; ++k) write_out(m.C, row * m.ldc + n0 + k, acc[k], alpha, beta);

@@ -114,9 +114,24 @@ __device__ __forceinline__ void accumulate_span(A& acc, const I* __restrict__ id
 #pragma unroll
     for (int u = 0; u < U; ++u) acc.add_prod(a[u], xv[u]);
   }
-  for (; j < e; j += STRIDE) {
-    const int64_t c = ldg_index(idx, j);
-    acc.add_prod(vals.template get<C>(j), load_as<C>(x, c));
+  if (j < e) {
+    // the last, partial batch (a row shorter than U * STRIDE is only this):
+    // all of its loads in flight at once, entries past e masked; the
+    // products are added in the same order as one at a time
+    int64_t col[U];
+    C a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = j + u * STRIDE < e;
+      col[u] = ok ? ldg_index(idx, j + u * STRIDE) : 0;
+      a[u] = ok ? vals.template get<C>(j + u * STRIDE) : zero<C>();
+    }
+    C xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = j + u * STRIDE < e ? load_as<C>(x, col[u]) : zero<C>();
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j + u * STRIDE < e) acc.add_prod(a[u], xv[u]);
   }
 }
 
