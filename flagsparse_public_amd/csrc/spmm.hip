@@ -276,6 +276,95 @@ constexpr bool kHotSpmm = (std::is_same<V, float>::value || std::is_same<V, doub
 template <typename V, typename I, int G, int U>
 constexpr bool kGroupLoads = kHotSpmm<V, I> && G >= U && G % U == 0;
 
+// Lane groups narrower than a batch (G < U: few dense columns) take the
+// wavefront's 64 / G consecutive rows together: their entries are
+// contiguous, so the wavefront copies them into its LDS window with coalesced
+// loads (64 consecutive entries per instruction) and each group reads its
+// row's columns and values from there.  A wavefront whose rows hold more than
+// kMmStageMax entries (a long row among them) loads per group instead.
+template <typename V, typename I, int G, int U>
+constexpr bool kWaveStaged = kHotSpmm<V, I> && G < U;
+
+constexpr int kMmStage = 512;          // entries per wavefront window
+constexpr int kMmStageMax = 8 * kMmStage;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename A, int G, int VW, int U, typename V, typename I, typename P>
+__device__ __forceinline__ void spmm_staged_rows(const SpmmParams<V, I, P>& p, int64_t rb,
+                                                 int64_t row, int64_t n0) {
+  using C = typename A::compute_t;
+  __shared__ int32_t st_col[kMmWaves][kMmStage];
+  __shared__ C st_val[kMmWaves][kMmStage];
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  const int64_t wrow0 = rb * (kMmBlock / G) + (int64_t)wv * (kWave / G);
+  if (wrow0 >= p.n_rows) return;  // the whole wavefront
+  const int64_t wrow1 = wrow0 + kWave / G < p.n_rows ? wrow0 + kWave / G : p.n_rows;
+  const int64_t wb = (int64_t)p.ptr[wrow0], we = (int64_t)p.ptr[wrow1];
+  const bool live = row < wrow1;
+  const int64_t rs = live ? (int64_t)p.ptr[row] : we, re = live ? (int64_t)p.ptr[row + 1] : we;
+  const bool mine = live && re - rs <= p.short_max && n0 < p.n_dense;  // longer: chunked
+  A acc[VW];
+#pragma unroll
+  for (int k = 0; k < VW; ++k) acc[k].init();
+  if (we - wb > kMmStageMax) {
+    if (!mine) return;
+    mm_accumulate<VW, U>(acc, p.idx, p.val, p.B, p.ldb, n0, rs, re, 1);
+  } else {
+    int32_t* sc = st_col[wv];
+    C* sv = st_val[wv];
+    for (int64_t w0 = wb; w0 < we; w0 += kMmStage) {
+      const int64_t cnt = we - w0 < kMmStage ? we - w0 : kMmStage;
+#pragma unroll
+      for (int k = 0; k < kMmStage / kWave; ++k) {
+        const int i = k * kWave + lane;
+        if (i < cnt) {
+          sc[i] = (int32_t)p.idx[w0 + i];
+          sv[i] = load_as<C>(p.val, w0 + i);
+        }
+      }
+      wave_lds_sync();
+      const int64_t lo = rs > w0 ? rs : w0;
+      const int64_t hi = re < w0 + cnt ? re : w0 + cnt;
+      if (mine) {
+        for (int64_t j = lo; j < hi; j += U) {
+          int64_t col[U];
+          C a[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const bool ok = j + u < hi;
+            col[u] = ok ? (int64_t)sc[j + u - w0] : 0;
+            a[u] = ok ? sv[j + u - w0] : zero<C>();
+          }
+          C b[U][VW];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            if (j + u < hi) {
+              load_vec<C, VW>(b[u], p.B, col[u] * p.ldb + n0);
+            } else {
+#pragma unroll
+              for (int k = 0; k < VW; ++k) b[u][k] = zero<C>();
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (j + u < hi)
+#pragma unroll
+              for (int k = 0; k < VW; ++k) acc[k].add_prod(a[u], b[u][k]);
+        }
+      }
+      wave_lds_sync();  // the window is rewritten next
+    }
+    if (!mine) return;
+  }
+#pragma unroll
+  for (int k = 0; k < VW; ++k) write_out(p.C, row * p.ldc + n0 + k, acc[k].value(), p.alpha, p.beta);
+}
+
 template <typename V, typename A, typename I, typename P, int G, int VW, int U>
 __global__ __launch_bounds__(kMmBlock) void spmm_csr_kernel(SpmmParams<V, I, P> p) {
   const int64_t bid = blockIdx.x;
@@ -287,9 +376,13 @@ __global__ __launch_bounds__(kMmBlock) void spmm_csr_kernel(SpmmParams<V, I, P> 
   if (p.xcd_remap) sb = xcd_remap_block(sb, (int64_t)gridDim.x - p.n_chunk_blocks);
   const int64_t tile = sb % p.n_tiles, rb = sb / p.n_tiles;
   const int64_t row = rb * (kMmBlock / G) + threadIdx.x / G;
-  if (row >= p.n_rows) return;  // the whole group
   const int g = threadIdx.x % G;
   const int64_t n0 = tile * (G * VW) + (int64_t)g * VW;
+  if constexpr (kWaveStaged<V, I, G, U>) {
+    spmm_staged_rows<A, G, VW, U>(p, rb, row, n0);
+    return;
+  }
+  if (row >= p.n_rows) return;  // the whole group
   const int64_t rs = (int64_t)p.ptr[row], re = (int64_t)p.ptr[row + 1];
   if (re - rs > p.short_max) return;  // chunked elsewhere (the whole group)
   A acc[VW];
@@ -407,6 +500,9 @@ int launch_spmm(const fs_csr* M, const void* B, int64_t ldb, void* C, int64_t ld
   // a column tile wider than B would leave the lanes past n_dense idle
   int64_t tile = col_tile > 0 && col_tile < n_dense ? col_tile : n_dense;
   if (tile % vw != 0) vw = 1;
+  // narrow tiles keep at least 8 lanes per row (narrower loads, more lanes),
+  // so a row's entries are loaded a whole batch at a time by its group
+  while (vw > 1 && tile / vw < 8) vw >>= 1;
   int G = pow2_at_least((tile + vw - 1) / vw);
   if (vw == 4) return launch_spmm_g<V, A, I, P, 4>(G, M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
   if (vw == 2) return launch_spmm_g<V, A, I, P, 2>(G, M, B, ldb, C, ldc, n_dense, alpha, beta, pl, st);
