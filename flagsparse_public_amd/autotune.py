@@ -15,6 +15,7 @@ resume.
 
 from __future__ import annotations
 
+import dataclasses
 import os
 import statistics
 
@@ -23,7 +24,7 @@ import torch
 
 from .rowsplit import Schedule
 
-VARIANTS_VERSION = 3
+VARIANTS_VERSION = 4
 
 # SpMV (fp32, native accumulation) variants.  0-5: entry tiles (fixed runs of
 # entries), 3-5 with the columns cut into L2-sized slices when x is larger
@@ -143,13 +144,19 @@ def tune_spmv(prepared, x, variants=SPMV_VARIANTS, reps=7, warmup=2, inner=1):
 
 
 def tune_spmm(prepared, B, variants=SPMM_VARIANTS, reps=5, warmup=1, inner=1):
-    """Median us of every SpMM variant (native accumulation) on one matrix."""
-    from .sparse_operations.spmm_csr import _hip_spmm_csr_impl_opt_prepared
+    """Median us of every SpMM variant (native accumulation) on one matrix,
+    each launched as the library launches a schedule it chose itself (entries
+    in flight and column tile adjusted for B's dtype and width)."""
+    from .sparse_operations.spmm_csr import (_hip_spmm_csr_impl_opt_prepared,
+                                             _library_spmm_launch)
     C = torch.empty((prepared.n_rows, B.shape[1]), dtype=B.dtype, device=B.device)
+    n_dense = int(B.shape[1])
     times = []
     for sch in variants:
+        run, col_tile = _library_spmm_launch(sch, B.dtype, n_dense)
+        run = dataclasses.replace(run, col_tile=col_tile)
         times.append(time_launches(
-            lambda s=sch: _hip_spmm_csr_impl_opt_prepared(prepared, B, out=C, schedule=s),
+            lambda s=run: _hip_spmm_csr_impl_opt_prepared(prepared, B, out=C, schedule=s),
             reps, warmup, inner))
     return np.asarray(times)
 

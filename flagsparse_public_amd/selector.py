@@ -17,6 +17,7 @@ from __future__ import annotations
 import json
 import math
 import os
+import warnings
 
 import numpy as np
 import torch
@@ -209,9 +210,17 @@ def model_path(op: str) -> str:
 
 
 def load_default(op: str, device):
-    """The packaged model for op ("spmv" / "spmm64") or None if absent."""
+    """The packaged model for op ("spmv" / "spmm64"), or None if absent or
+    trained for another variant list (a warning says so; callers then use
+    the fixed rule)."""
     key = (op, str(device))
     if key not in _CACHE:
         path = model_path(op)
-        _CACHE[key] = Selector.load(path, device) if os.path.exists(path) else None
+        model = None
+        if os.path.exists(path):
+            try:
+                model = Selector.load(path, device)
+            except ValueError as exc:
+                warnings.warn(f"schedule selector not used: {exc}", RuntimeWarning, stacklevel=2)
+        _CACHE[key] = model
     return _CACHE[key]

@@ -55,7 +55,7 @@ def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=10
         n = min(n, max_n)
         indptr, cols, vals, params = synthetic.family_csr_torch(family, n, seed, device)
         nnz = int(vals.numel())
-        pv = prepare_spmv_csr(vals, cols, indptr, (n, n))
+        pv = prepare_spmv_csr(vals, cols, indptr, (n, n), schedule="default")
         stats = _lib.analyze_device(pv.csr_struct(), device)
         feats = _lib.features(stats, n, n, nnz, device).double().cpu().numpy()
         g = torch.Generator(device=device)
@@ -66,7 +66,7 @@ def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=10
         if n not in dense_cache:
             dense_cache.clear()
             dense_cache[n] = torch.rand((n, 64), generator=g, device=device) * 2 - 1
-        pm = prepare_spmm_csr_opt(vals, cols, indptr, (n, n))
+        pm = prepare_spmm_csr_opt(vals, cols, indptr, (n, n), schedule="default")
         tm = tune_spmm(pm, dense_cache[n], reps=max(3, reps - 2))
         dbs["spmm64"].add(seed, family, feats, tm, n, nnz)
         del pv, pm, indptr, cols, vals, x
