@@ -19,7 +19,7 @@ dominant kernel is the compact-row SpMV; its own algorithmic bytes are
 for NC compact rows, and its average duration is timed here with HIP events
 on the launch stream, in a loop of its own after the timed steps.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--path opt|default]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--path opt|default] [--dtype f32|f64]
 """
 
 from __future__ import annotations
@@ -198,7 +198,7 @@ def vendor_spmv(d_vals, d_cols, d_ip, d_x, n, nbytes, steps):
     from flagsparse_public_amd import vendor
     if not vendor.available():
         return {"library": "rocSPARSE", "unavailable": vendor.unavailable_reason()}
-    y = torch.empty(n, device=d_x.device)
+    y = torch.empty(n, dtype=d_x.dtype, device=d_x.device)
     v = vendor.VendorSpmv(d_vals, d_cols, d_ip, (n, n), d_x, y, "adaptive")
     us = _event_time(v.run, steps, torch.cuda.current_stream(d_x.device))
     _log(f"rocSPARSE adaptive: {us:.1f} us/step")
@@ -246,6 +246,54 @@ def bench_spmm(args):
         v = vendor.VendorSpmm(*d, (n, n), B, C)
         vus = _event_time(v.run, args.steps, stream)
         res["vendor_baseline"] = {"library": "rocSPARSE CSR SpMM", "us": round(vus, 2),
+                                  "GB/s": round(nbytes / vus / 1e3, 1)}
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_sddmm(args):
+    """out[p] = <x[row p], y[col p]> over a 1M x 1M pattern with Poisson(16)
+    row lengths (16.8M entries), x and y [1M, K] row-major, K = --n-dense.
+    GB/s over the gather-counted bytes nnz*(4 row id + 4 col + s out) +
+    2*nnz*K*s (both rows read per entry); compulsory bytes count x and y once.
+    The rocSPARSE SDDMM on the same inputs is the vendor line."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, ROOT)
+    from flagsparse_public_amd import flagsparse_sddmm_csr, prepare_sddmm_csr, synthetic, vendor
+    dev = torch.device("cuda", 0)
+    dt = torch.float32 if args.dtype == "f32" else torch.float64
+    n, K = 1 << 20, args.n_dense
+    ip, cols, _ = synthetic.poisson_csr(n, 16.0, 1)
+    nnz = int(ip[-1])
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    x = torch.rand((n, K), generator=g, device=dev, dtype=dt) * 2 - 1
+    y = torch.rand((n, K), generator=g, device=dev, dtype=dt) * 2 - 1
+    p = prepare_sddmm_csr(torch.from_numpy(cols).to(dev), torch.from_numpy(ip).to(dev), (n, n),
+                          k_hint=K)
+    out = torch.empty(nnz, dtype=dt, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    us = _event_time(lambda: flagsparse_sddmm_csr(x=x, y=y, prepared=p, out=out), args.steps,
+                     stream)
+    s_v = x.element_size()
+    nbytes = nnz * (4 + 4 + s_v) + 2 * nnz * K * s_v
+    gbs = nbytes / us / 1e3
+    res = {"metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": 3, "ms_per_step": round(us / 1e3, 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.dtype,
+           "data": "synthetic",
+           "config": {"workload": "sddmm_poisson_1m", "n_rows": n, "nnz": nnz, "k_dim": K,
+                      "compulsory_bytes": nnz * (4 + 4 + s_v) + 2 * n * K * s_v},
+           "roofline": {"bound": "hbm", "kernel": "sddmm_kernel", "achieved": round(gbs, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "traffic": None, "algorithmic_bytes_per_launch": nbytes,
+                        "avg_launch_us": round(us, 3)}}
+    if not args.no_vendor and vendor.available():
+        vout = torch.empty(nnz, dtype=dt, device=dev)
+        v = vendor.VendorSddmm(p.indices, p.indptr, p.shape, x, y, vout)
+        vus = _event_time(v.run, args.steps, stream)
+        res["vendor_baseline"] = {"library": "rocSPARSE SDDMM", "us": round(vus, 2),
                                   "GB/s": round(nbytes / vus / 1e3, 1)}
     print(json.dumps(res), flush=True)
     return 0
@@ -325,11 +373,14 @@ def main(argv=None):
                     help="skip the split-precision and default-path lines")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE line")
     ap.add_argument("--workload", default="spmv_powerlaw_4m",
-                    choices=["spmv_powerlaw_4m", "spmm_banded_1m", "batched_512"],
-                    help="the SpMV headline workload (default), the banded SpMM or the batch")
-    ap.add_argument("--n-dense", type=int, default=64, help="spmm_banded_1m: columns of B")
+                    choices=["spmv_powerlaw_4m", "spmm_banded_1m", "batched_512",
+                             "sddmm_poisson_1m"],
+                    help="the SpMV headline workload (default), the banded SpMM, the batch "
+                         "or the SDDMM")
+    ap.add_argument("--n-dense", type=int, default=64,
+                    help="spmm_banded_1m: columns of B; sddmm_poisson_1m: K")
     ap.add_argument("--dtype", choices=["f32", "f64"], default="f32",
-                    help="spmm_banded_1m: value type")
+                    help="value type of the workload (split-precision modes: fp32 only)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run device copies of this many MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -343,6 +394,8 @@ def main(argv=None):
         return bench_spmm(args)
     if args.workload == "batched_512":
         return bench_batched(args)
+    if args.workload == "sddmm_poisson_1m":
+        return bench_sddmm(args)
 
     import numpy as np
     import torch
@@ -355,10 +408,12 @@ def main(argv=None):
         return 1
     dev = torch.device("cuda", 0)
     t0 = time.time()
-    ip, cols, vals = synthetic.spmv_powerlaw_4m(n=args.n)
+    npd = np.float32 if args.dtype == "f32" else np.float64
+    s_v = np.dtype(npd).itemsize
+    ip, cols, vals = synthetic.spmv_powerlaw_4m(n=args.n, dtype=npd)
     n = ip.size - 1
     nnz = int(ip[-1])
-    x = synthetic.dense_uniform(202, n, np.float32)
+    x = synthetic.dense_uniform(202, n, npd)
     _log(f"generated n={n} nnz={nnz} max_row={int(np.diff(ip).max())} in {time.time() - t0:.1f}s")
     d_ip = torch.from_numpy(ip.astype(np.int32)).to(dev)
     d_cols = torch.from_numpy(cols).to(dev)
@@ -366,7 +421,7 @@ def main(argv=None):
     d_x = torch.from_numpy(x).to(dev)
     prepared = prepare_spmv_csr(d_vals, d_cols, d_ip, (n, n))
     use_opt = args.path == "opt"
-    y = torch.empty(n, dtype=torch.float32, device=dev)
+    y = torch.empty(n, dtype=d_vals.dtype, device=dev)
 
     def step():
         return flagsparse_spmv_csr(x=d_x, prepared=prepared, use_opt=use_opt)
@@ -387,7 +442,7 @@ def main(argv=None):
     wall_ms = (time.perf_counter() - w0) * 1e3
     ev_ms = ev0.elapsed_time(ev1)
     per_launch_ms = ev_ms / args.steps
-    nbytes = spmv_bytes(n, nnz)
+    nbytes = spmv_bytes(n, nnz, s_val=s_v, s_x=s_v, s_y=s_v)
     gbs = nbytes / (per_launch_ms * 1e-3) / 1e9
     _log(f"{args.steps} steps: events {ev_ms:.3f} ms ({per_launch_ms * 1e3:.2f} us/launch), "
          f"wall {wall_ms:.3f} ms, {gbs:.1f} GB/s")
@@ -403,7 +458,7 @@ def main(argv=None):
         def kernel():
             _lib.spmv(sliced.csr, d_x, None if sliced.sums else sliced.z, 1.0, 0.0, kplan, dev)
         nc = sliced.n_compact
-        kbytes = (nc + 1) * 4 + nnz * 12 + nc * sliced.z.element_size()
+        kbytes = (nc + 1) * 4 + nnz * (4 + 2 * s_v) + nc * sliced.z.element_size()
         kname = "spmv_entry_kernel (compact rows) + carry fix-up"
     else:
         kernel = step
@@ -434,7 +489,7 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.dtype,
         "data": "synthetic",
         "config": {
             "workload": "spmv_powerlaw_4m",
@@ -446,7 +501,7 @@ def main(argv=None):
                          "compact_rows": nc,
                          "chosen_by": prepared.opt_buckets[0].get("chosen_by")},
             "gflops": round(2.0 * nnz / (per_launch_ms * 1e-3) / 1e9, 2),
-            "compulsory_bytes": (n + 1) * 4 + nnz * 8 + n * 4 + n * 4,
+            "compulsory_bytes": (n + 1) * 4 + nnz * (4 + s_v) + n * s_v + n * s_v,
         },
         "roofline": {
             "bound": "hbm",
@@ -455,7 +510,7 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(kgbs / HBM_PEAK_GBS, 4),
-            "traffic": _traffic_from_profile(),
+            "traffic": _traffic_from_profile() if args.dtype == "f32" else None,
             "algorithmic_bytes_per_launch": kbytes,
             "avg_launch_us": round(kernel_us, 3),
             "step_us": round(per_launch_ms * 1e3, 3),
@@ -465,7 +520,7 @@ def main(argv=None):
     if not args.no_vendor:
         result["vendor_baseline"] = vendor_spmv(d_vals, d_cols, d_ip, d_x, n, nbytes, args.steps)
     modes = None
-    if not args.no_modes:
+    if not args.no_modes and args.dtype == "f32":  # split storage is an fp32 format
         modes = measure_modes(d_vals, d_cols, d_ip, d_x, n, nnz, prepared, use_opt,
                               max(args.steps // 4, 10))
     if not args.no_cpu:
