@@ -81,7 +81,7 @@ def cpu_baseline(ip, cols, vals, x, nbytes, budget_s=10.0):
     res = {"value": round(nbytes / sec / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
            "sample": f"{reps} whole-matrix passes of the C oracle (fp64 accumulate, "
                      f"sequential, 1 thread), {sec * 1e3:.1f} ms each; GB/s over the same "
-                     f"fp32 algorithmic bytes",
+                     f"algorithmic bytes as the GPU line",
            "host_cpus": os.cpu_count(), "library_cpu_paths": library_cpu_paths(ip, cols, vals, x,
                                                                                nbytes)}
     return res, y64
@@ -110,7 +110,8 @@ def library_cpu_paths(ip, cols, vals, x, nbytes, reps=5):
         import scipy.sparse as sp
         A = sp.csr_matrix((vals, cols, ip), shape=(n, n))
         sec = timed(lambda: A @ x)
-        out.append({"library": "scipy csr_matrix @ x (fp32, serial)", "ms": round(sec * 1e3, 2),
+        out.append({"library": f"scipy csr_matrix @ x ({vals.dtype}, serial)",
+                    "ms": round(sec * 1e3, 2),
                     "GB/s": round(nbytes / sec / 1e9, 3), "threads": 1})
         del A
     except Exception as exc:  # reported, not fatal: a baseline only
@@ -127,7 +128,7 @@ def library_cpu_paths(ip, cols, vals, x, nbytes, reps=5):
                                         torch.from_numpy(vals), size=(n, n))
         xt = torch.from_numpy(x)
         sec = timed(lambda: torch.mv(A, xt))
-        out.append({"library": "torch CPU CSR mv (fp32)", "ms": round(sec * 1e3, 2),
+        out.append({"library": f"torch CPU CSR mv ({vals.dtype})", "ms": round(sec * 1e3, 2),
                     "GB/s": round(nbytes / sec / 1e9, 3), "threads": torch.get_num_threads()})
         del A
     except Exception as exc:
