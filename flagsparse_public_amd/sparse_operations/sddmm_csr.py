@@ -183,17 +183,22 @@ def _run_sddmm_prepared(prepared, x, y, data, alpha, beta, out, allow_fallback=F
             "out_dtype": str(out.dtype).replace("torch.", "")}
     k_dim = int(x.shape[1])
     meta["block_k"], meta["num_warps"] = _resolve_sddmm_launch_config(k_dim)
-    lib = _lib.load()  # a missing library raises here, before any fallback
-    del lib
+    _lib.load()  # a missing library raises here, before any fallback
+    # the kernel writes entry p at element p: a strided out gets a dense
+    # staging buffer, copied back
+    target = out if out.is_contiguous() else torch.empty(out.shape, dtype=out.dtype,
+                                                          device=out.device)
     csr = _lib.FsCsr(value_type=_lib.value_code(x.dtype), index_type=_lib.FS_I32,
                      ptr_type=_lib.FS_I64, reserved=0, n_rows=prepared.n_rows,
                      n_cols=prepared.n_cols, nnz=prepared.nnz, row_ptr=_lib.ptr(prepared.indptr),
                      col_idx=_lib.ptr(prepared.indices),
                      values=_lib.ptr(data) if data is not None else None)
     try:
-        _lib.sddmm(csr, prepared.row_ids, x, y, k_dim, out, _lib.value_code(out_dtype),
+        _lib.sddmm(csr, prepared.row_ids, x, y, k_dim, target, _lib.value_code(out_dtype),
                    _lib.FS_ACC_WIDE if wide else _lib.FS_ACC_NATIVE, alpha,
                    beta if data is not None else 0.0, x.device)
+        if target is not out:
+            out.copy_(target)
     except Exception:
         if not allow_fallback:
             raise

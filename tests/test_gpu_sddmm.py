@@ -121,3 +121,23 @@ def test_sddmm_fp32_native_into_fp64_out(orc):
     assert o.dtype == torch.float64 and m["acc_dtype"] == "float32"
     err = np.abs(o.cpu().numpy() - want) / np.maximum(cond, 1e-300)
     assert err.max() <= 2e-6 and err.max() > 0  # fp32 accumulation, not fp64
+
+
+def test_sddmm_strided_out_and_prepared_reuse(orc):
+    """A strided out view is filled in place (same object returned); one
+    prepared pattern serves several x / y of different K."""
+    ip, cols, _ = synthetic.poisson_csr(500, 7.0, 9)
+    n = ip.size - 1
+    p = prepare_sddmm_csr(torch.from_numpy(cols).to(DEV), torch.from_numpy(ip).to(DEV), (n, n))
+    rng = np.random.default_rng(21)
+    for k in (8, 33):
+        x = rng.uniform(-1, 1, (n, k)).astype(np.float32)
+        y = rng.uniform(-1, 1, (n, k)).astype(np.float32)
+        want, cond = orc.sddmm(ip, cols, x, y)
+        big = torch.zeros(2 * cols.size, device=DEV)
+        view = big[::2]
+        got = flagsparse_sddmm_csr(x=torch.from_numpy(x).to(DEV), y=torch.from_numpy(y).to(DEV),
+                                   prepared=p, out=view)
+        assert got is view
+        assert torch.count_nonzero(big[1::2]) == 0  # only the view's elements written
+        assert (np.abs(view.cpu().numpy() - want) / np.maximum(cond, 1e-300)).max() <= 2e-6
