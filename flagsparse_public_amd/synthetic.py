@@ -299,10 +299,15 @@ def family_csr_torch(family, n, seed, device, params=None):
     return indptr, cols, vals, p
 
 
+SELECTOR_LARGE_SEED0 = 9000  # seeds from here on draw the large-matrix supplement
+
+
 def selector_matrix_spec(seed):
     """(family, n) of selector matrix `seed`: family by seed mod 5, n
-    log-uniform in [1e4, 2e6]."""
+    log-uniform in [1e4, 2e6] (seeds below SELECTOR_LARGE_SEED0) or in
+    [2e6, 5e6] (the large-matrix supplement)."""
     rng = np.random.Generator(np.random.PCG64(10_000_000 + seed))
     family = SELECTOR_FAMILIES[seed % len(SELECTOR_FAMILIES)]
-    n = int(round(float(np.exp(rng.uniform(np.log(1e4), np.log(2e6))))))
+    lo, hi = (2e6, 5e6) if seed >= SELECTOR_LARGE_SEED0 else (1e4, 2e6)
+    n = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
     return family, n

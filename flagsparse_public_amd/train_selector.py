@@ -4,8 +4,9 @@
         [--n-test 200] [--max-n 2000000] [--resume]
 
 Matrices: seeds 5000.. (training) and 7000.. (held out), five families
-(``synthetic.SELECTOR_FAMILIES``) with n log-uniform in [1e4, 2e6], fp32
-values, generated on the GPU.  For every matrix all SpMV variants and all
+(``synthetic.SELECTOR_FAMILIES``) with n log-uniform in [1e4, 2e6], plus a
+large-matrix supplement with n log-uniform in [2e6, 5e6] (seeds 9000..
+training, 9500.. held out), fp32 values, generated on the GPU.  For every matrix all SpMV variants and all
 64-column SpMM variants are timed (``autotune``); the selectors are trained
 on the training rows with the library's HIP MLP kernels; on the held-out
 matrices the on-device selection is compared with exhaustive autotuning
@@ -36,6 +37,8 @@ from .sparse_operations.spmv_csr import prepare_spmv_csr
 
 TRAIN_SEED0 = 5000
 TEST_SEED0 = 7000
+LARGE_TRAIN_SEED0 = 9000
+LARGE_TEST_SEED0 = 9500
 
 
 def _log(msg):
@@ -72,7 +75,7 @@ def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=10
         del pv, pm, indptr, cols, vals, x
         if save is not None and (k + 1) % save_every == 0:
             save()
-        if (k + 1) % 20 == 0 or k + 1 == len(seeds):
+        if (k + 1) % 20 == 0 or k + 1 == len(seeds) or n >= 1_000_000:
             _log(f"{label} {k + 1}/{len(seeds)} matrices, {time.time() - t0:.0f}s "
                  f"(last: {family} n={n} nnz={nnz}, spmv best {tv.min():.1f}us, "
                  f"spmm best {tm.min():.1f}us)")
@@ -127,7 +130,10 @@ def main(argv=None):
     ap.add_argument("--out", required=True)
     ap.add_argument("--n-train", type=int, default=2000)
     ap.add_argument("--n-test", type=int, default=200)
-    ap.add_argument("--max-n", type=int, default=2_000_000)
+    ap.add_argument("--n-large", type=int, default=100,
+                    help="training matrices of the large supplement (n in [2e6, 5e6])")
+    ap.add_argument("--n-large-test", type=int, default=10)
+    ap.add_argument("--max-n", type=int, default=5_000_000)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--epochs", type=int, default=4000)
     ap.add_argument("--resume", action="store_true", help="reuse DIR/*_db.npz if present")
@@ -145,7 +151,9 @@ def main(argv=None):
     deadline = t0 + args.time_budget if args.time_budget > 0 else None
     complete = True
     # held-out matrices first, so a partial collection can already be judged
-    for split, seed0, count in (("test", TEST_SEED0, args.n_test), ("train", TRAIN_SEED0, args.n_train)):
+    splits = (("test", [(TEST_SEED0, args.n_test), (LARGE_TEST_SEED0, args.n_large_test)]),
+              ("train", [(TRAIN_SEED0, args.n_train), (LARGE_TRAIN_SEED0, args.n_large)]))
+    for split, ranges in splits:
         have = {}
         if args.resume and all(os.path.exists(p) for p in paths[split].values()):
             have = {op: TimingDB.load(p) for op, p in paths[split].items()}
@@ -153,7 +161,7 @@ def main(argv=None):
             have = {"spmv": TimingDB("spmv", len(SPMV_VARIANTS), name),
                     "spmm64": TimingDB("spmm64", len(SPMM_VARIANTS), name)}
         done = set(have["spmv"].ids)
-        seeds = [s for s in range(seed0, seed0 + count) if s not in done]
+        seeds = [s for seed0, count in ranges for s in range(seed0, seed0 + count) if s not in done]
         def save(have=have, split=split):
             for op, db in have.items():
                 db.save(paths[split][op])
