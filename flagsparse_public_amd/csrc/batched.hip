@@ -127,13 +127,22 @@ __device__ void bat_spmm_rows(const BatchMatrix& m, int64_t r0, int64_t r1, doub
           for (int k = 0; k < VW; ++k) acc[k] = __builtin_fmaf(a[u], b[u][k], acc[k]);
       }
       if (j < re) {  // the last, partial batch: its loads in flight together
-        int64_t col[4];
+        int32_t ci[4];
         float a[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const bool ok = j + u < re;
-          col[u] = ok ? m.col[j + u] : 0;
+          ci[u] = ok ? m.col[j + u] : 0;
           a[u] = ok ? m.val[j + u] : 0.f;
+        }
+        // the empty asm keeps each index's widening (and the wait for its
+        // load) out of the masked load's branch, so all four loads are issued
+        // before the first wait
+        int64_t col[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          asm volatile("" : "+v"(ci[u]));
+          col[u] = ci[u];
         }
         float b[4][VW];
 #pragma unroll
@@ -156,8 +165,23 @@ __device__ void bat_spmm_rows(const BatchMatrix& m, int64_t r0, int64_t r1, doub
 #pragma unroll
             for (int k = 0; k < VW; ++k) acc[k] = __builtin_fmaf(a[u], b[u][k], acc[k]);
       }
+      if constexpr (VW == 4) {  // write_out's arithmetic, one 16-byte store
+        float* c = m.C + row * m.ldc + n0;
+        float v[4];
 #pragma unroll
-      for (int k = 0; k < VW; ++k) write_out(m.C, row * m.ldc + n0 + k, acc[k], alpha, beta);
+        for (int k = 0; k < 4; ++k) v[k] = alpha == 1.0 ? acc[k] : scale(acc[k], alpha);
+        if (beta != 0.0) {
+          const float4 o = *reinterpret_cast<const float4*>(c);
+          v[0] = add(v[0], scale(o.x, beta));
+          v[1] = add(v[1], scale(o.y, beta));
+          v[2] = add(v[2], scale(o.z, beta));
+          v[3] = add(v[3], scale(o.w, beta));
+        }
+        *reinterpret_cast<float4*>(c) = make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) write_out(m.C, row * m.ldc + n0 + k, acc[k], alpha, beta);
+      }
     }
   }
 }
@@ -280,7 +304,7 @@ extern "C" int fs_batch_pack(const fs_batch_entry* entries, int64_t n_entries, v
     m.ptr64 = A.ptr_type == FS_I64;
     int vw = 1;
     if (e.n_dense > 0 && e.n_dense % 4 == 0 && e.ldb % 4 == 0 && e.ldc % 4 == 0 &&
-        ((uintptr_t)e.B % 16) == 0)
+        ((uintptr_t)e.B % 16) == 0 && ((uintptr_t)e.C % 16) == 0)
       vw = 4;
     m.mm_vw = vw;
     m.mm_lanes = pow2_ceil_i((e.n_dense + vw - 1) / vw);

@@ -89,9 +89,14 @@ __global__ __launch_bounds__(kCombineRows) void sliced_combine_mask_kernel(
   __shared__ int64_t base[kGroup];
   const int64_t b = blockIdx.x;
   const int tid = threadIdx.x;
+  const int64_t row = b * kCombineRows + tid;
+  S acc = zero<S>();
+  if (n_compact == 0) {  // uniform: no partials to read
+    if (row < n_rows) write_out(y, row, acc, alpha, beta);
+    return;
+  }
   const int w = tid / 32, bit = tid % 32;
   const uint32_t below = bit == 0 ? 0u : (0xffffffffu >> (32 - bit));
-  S acc = zero<S>();
   for (int k0 = 0; k0 < n_slices; k0 += kGroup) {
     if (k0 > 0) __syncthreads();  // the previous group's words are consumed
     if (tid < kGroup * kMaskWords) {
@@ -119,14 +124,16 @@ __global__ __launch_bounds__(kCombineRows) void sliced_combine_mask_kernel(
       const uint32_t mine = words[u][w];
       const int64_t i = base[u] + before[u][w] + __popc(mine & below);
       const bool has = ((mine >> bit) & 1u) && i < n_compact;  // a bad mask reads nothing
-      zv[u] = has ? load_as<C>(z, i) : zero<C>();
+      // every lane loads (absent rows read partial 0) so that the kGroup
+      // loads stay unconditional and are all in flight before the first add;
+      // a load under a per-lane branch is waited for inside the branch
+      zv[u] = load_as<C>(z, has ? i : 0);
       present |= (unsigned)has << u;
     }
 #pragma unroll
     for (int u = 0; u < kGroup; ++u)
       if ((present >> u) & 1u) acc = add(acc, widen(zv[u]));
   }
-  const int64_t row = b * kCombineRows + tid;
   if (row < n_rows) write_out(y, row, acc, alpha, beta);
 }
 

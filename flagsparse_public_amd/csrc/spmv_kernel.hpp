@@ -118,17 +118,23 @@ __device__ __forceinline__ void accumulate_span(A& acc, const I* __restrict__ id
     // the last, partial batch (a row shorter than U * STRIDE is only this):
     // all of its loads in flight at once, entries past e masked; the
     // products are added in the same order as one at a time
-    int64_t col[U];
+    I ci[U];
     C a[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const bool ok = j + u * STRIDE < e;
-      col[u] = ok ? ldg_index(idx, j + u * STRIDE) : 0;
+      ci[u] = ok ? idx[j + u * STRIDE] : I(0);
       a[u] = ok ? vals.template get<C>(j + u * STRIDE) : zero<C>();
     }
+    // the compiler would otherwise widen each index inside its masked load's
+    // branch and wait for that load there, one load at a time; the empty asm
+    // keeps the raw indices opaque until every load of the batch is issued
+#pragma unroll
+    for (int u = 0; u < U; ++u) asm volatile("" : "+v"(ci[u]));
     C xv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) xv[u] = j + u * STRIDE < e ? load_as<C>(x, col[u]) : zero<C>();
+    for (int u = 0; u < U; ++u)
+      xv[u] = j + u * STRIDE < e ? load_as<C>(x, (int64_t)ci[u]) : zero<C>();
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (j + u * STRIDE < e) acc.add_prod(a[u], xv[u]);
@@ -455,10 +461,18 @@ __device__ __forceinline__ void entry_fetch(const SpmvParams<VS, V, I, P>& p, in
   f.r0 = r0;
   f.r1 = r1;
   f.nb0 = (int)(r1 - r0 < kSegRows ? r1 - r0 : kSegRows);
+  // Row pointers: a batch wholly past the owned rows is skipped by the whole
+  // block (nb0 is uniform); inside a batch, lanes past nb0 re-read the last
+  // owned pointer (r0 + nb0 <= r1 <= n_rows) instead of branching per lane --
+  // the compiler waits for a load under a per-lane branch inside the branch,
+  // which would serialise these loads with each other and with the entry
+  // loads below.
+  constexpr int kPv = EntryTile<I, C, E>::kPv;
+  P pr[kPv];
 #pragma unroll
-  for (int k = 0; k < EntryTile<I, C, E>::kPv; ++k) {
+  for (int k = 0; k < kPv; ++k) {
     const int i = k * kBlock + tid;
-    f.pv[k] = i <= f.nb0 ? (int64_t)p.ptr[r0 + i] : 0;
+    pr[k] = k * kBlock <= f.nb0 ? p.ptr[r0 + (i <= f.nb0 ? i : f.nb0)] : P(0);
   }
   f.vec = VEC && f.len == T && (f.e0 & 3) == 0;
   if (f.vec) {
@@ -492,6 +506,14 @@ __device__ __forceinline__ void entry_fetch(const SpmvParams<VS, V, I, P>& p, in
       f.col[u] = o < f.len ? p.idx[j] : I(0);
       f.a[u] = o < f.len ? p.vals.template get<C>(j) : zero<C>();
     }
+  }
+  // the row pointers are widened only now that the entry loads are issued;
+  // the empty asm keeps the widening (and its wait) from moving up into the
+  // row-pointer branches
+#pragma unroll
+  for (int k = 0; k < kPv; ++k) {
+    asm volatile("" : "+v"(pr[k]));
+    f.pv[k] = (int64_t)pr[k];
   }
 }
 

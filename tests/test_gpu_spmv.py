@@ -223,18 +223,20 @@ def test_sliced_combine_row_masks_equal_row_ids(dtype, use_opt):
     """The combine's two ways to find a row's partials -- presence masks with
     popcounts, or the compact-row ids -- give the same bits (fp64 row sums of
     the default fp32 path, native partials otherwise), including 24 slices
-    (three groups of 8) and alpha / beta."""
+    (three groups of 8), a row count that ends inside a 32-row mask word, and
+    alpha / beta."""
     from flagsparse_public_amd import _lib
     npd = {torch.float32: np.float32, torch.float64: np.float64,
            torch.complex64: np.complex64}[dtype]
-    ip, cols, vals = _powerlaw(30000, 5, 20000, np.float64)
+    ip, cols, vals = _powerlaw(30001, 5, 20000, np.float64)
     n = ip.size - 1
+    assert n % 32 != 0
     rng = np.random.default_rng(8)
     if dtype.is_complex:
         vals = vals + 1j * rng.uniform(-1, 1, vals.size)
     vals = vals.astype(npd)
     x = rng.uniform(-1, 1, n).astype(npd)
-    for slice_cols in (4000, 1250):
+    for slice_cols in (4000, 1251):
         sch = Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=slice_cols)
         p = prepare_spmv_csr(to_dev(vals), to_dev(cols), to_dev(ip), (n, n), schedule=sch)
         flagsparse_spmv_csr(x=to_dev(x), prepared=p, use_opt=use_opt)  # fills the partials
@@ -242,6 +244,7 @@ def test_sliced_combine_row_masks_equal_row_ids(dtype, use_opt):
         assert sliced
         s = sliced[-1]
         assert s.n_slices == (8 if slice_cols == 4000 else 24)
+        assert s.desc.row_mask
         no_mask = _lib.FsSlices(**{f: getattr(s.desc, f) for f, _ in _lib.FsSlices._fields_})
         no_mask.row_mask = None
         y0 = to_dev(rng.uniform(-1, 1, n).astype(npd))
