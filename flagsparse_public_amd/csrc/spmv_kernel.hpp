@@ -58,6 +58,17 @@ struct DirectVals {
   }
   static constexpr bool kRowScale = false;
   __device__ __forceinline__ float row_scale(int64_t) const { return 1.f; }
+  // raw stored value and its conversion, for loads that must stay
+  // unconverted until a whole batch is issued (accumulate_span)
+  static constexpr bool kRaw = std::is_arithmetic<V>::value;
+  using raw_t = V;
+  __device__ __forceinline__ V raw(int64_t j) const { return v[j]; }
+  template <typename C> __device__ __forceinline__ C conv(V r) const { return (C)r; }
+};
+
+template <typename VS, typename = void> struct HasRaw { static constexpr bool value = false; };
+template <typename VS> struct HasRaw<VS, std::void_t<decltype(VS::kRaw)>> {
+  static constexpr bool value = VS::kRaw;
 };
 
 template <typename VS, typename V, typename I, typename P>
@@ -117,7 +128,36 @@ __device__ __forceinline__ void accumulate_span(A& acc, const I* __restrict__ id
   if (j < e) {
     // the last, partial batch (a row shorter than U * STRIDE is only this):
     // all of its loads in flight at once, entries past e masked; the
-    // products are added in the same order as one at a time
+    // products are added in the same order as one at a time.  Each masked
+    // load's value is kept raw and opaque (empty asm) until the batch is
+    // issued: a conversion of it (index widening, fp32 -> fp64) would be
+    // placed inside the load's branch together with a wait for the load,
+    // serialising the batch.
+    if constexpr (HasRaw<VS>::value && std::is_arithmetic<V>::value) {
+      using R = typename VS::raw_t;
+      I ci[U];
+      R ar[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const bool ok = j + u * STRIDE < e;
+        ci[u] = ok ? idx[j + u * STRIDE] : I(0);
+        ar[u] = ok ? vals.raw(j + u * STRIDE) : R(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) asm volatile("" : "+v"(ci[u]));
+      V xr[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xr[u] = j + u * STRIDE < e ? x[(int64_t)ci[u]] : V(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        asm volatile("" : "+v"(ar[u]));
+        asm volatile("" : "+v"(xr[u]));
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j + u * STRIDE < e) acc.add_prod(vals.template conv<C>(ar[u]), (C)xr[u]);
+      return;
+    }
     I ci[U];
     C a[U];
 #pragma unroll
@@ -126,9 +166,6 @@ __device__ __forceinline__ void accumulate_span(A& acc, const I* __restrict__ id
       ci[u] = ok ? idx[j + u * STRIDE] : I(0);
       a[u] = ok ? vals.template get<C>(j + u * STRIDE) : zero<C>();
     }
-    // the compiler would otherwise widen each index inside its masked load's
-    // branch and wait for that load there, one load at a time; the empty asm
-    // keeps the raw indices opaque until every load of the batch is issued
 #pragma unroll
     for (int u = 0; u < U; ++u) asm volatile("" : "+v"(ci[u]));
     C xv[U];
