@@ -98,6 +98,46 @@ __device__ __forceinline__ void load_vec(C (&out)[VW], const S* __restrict__ p, 
   }
 }
 
+// B rows of a masked batch: b[u] = B[col[u] * ldb + n0 .. + VW) for the u
+// whose bit is set in `ok` (others zero).  When the compute type differs from
+// the stored one (fp32 B accumulated in fp64), the rows are loaded raw and kept
+// opaque (empty asm) until every load of the batch is issued, then widened: a
+// widening placed inside a masked load's branch brings a wait for that load
+// with it and serialises the batch.
+template <int VW, int U, typename C, typename V>
+__device__ __forceinline__ void gather_masked(C (&b)[U][VW], const V* __restrict__ B, int64_t ldb,
+                                              const int64_t (&col)[U], int64_t n0, unsigned ok) {
+  if constexpr (std::is_same<C, V>::value || !std::is_arithmetic<V>::value) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if ((ok >> u) & 1u) {
+        load_vec<C, VW>(b[u], B, col[u] * ldb + n0);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) b[u][k] = zero<C>();
+      }
+    }
+  } else {
+    V r[U][VW];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if ((ok >> u) & 1u) {
+        load_vec<V, VW>(r[u], B, col[u] * ldb + n0);
+      } else {
+#pragma unroll
+        for (int k = 0; k < VW; ++k) r[u][k] = V(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < VW; ++k) {
+        asm volatile("" : "+v"(r[u][k]));
+        b[u][k] = (C)r[u][k];
+      }
+  }
+}
+
 // C[off .. off + VW) = alpha * acc + beta * C (C read only when beta is
 // nonzero): write_out's arithmetic, with one vector load / store where
 // load_vec has a vector form (the launch keeps C aligned whenever VW > 1) and
@@ -172,21 +212,15 @@ __device__ __forceinline__ void mm_accumulate(A (&acc)[VW], const I* __restrict_
     // out of the masked load's branch, so the batch's loads are all issued
     // before the first wait
     int64_t col[U];
+    unsigned ok = 0;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       asm volatile("" : "+v"(ci[u]));
       col[u] = (int64_t)ci[u];
+      ok |= (unsigned)(j + u * step < e) << u;
     }
     C b[U][VW];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (j + u * step < e) {
-        load_vec<C, VW>(b[u], B, col[u] * ldb + n0);
-      } else {
-#pragma unroll
-        for (int k = 0; k < VW; ++k) b[u][k] = zero<C>();
-      }
-    }
+    gather_masked<VW, U>(b, B, ldb, col, n0, ok);
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (j + u * step < e)
@@ -231,16 +265,11 @@ __device__ __forceinline__ void mm_accumulate_group(A (&acc)[VW], int32_t* slot_
           col[u] = (int64_t)slot_col[uu];
           a[u] = slot_val[uu];
         }
+        unsigned ok = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) ok |= (unsigned)(u0 + u < cnt) << u;
         C b[U][VW];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (u0 + u < cnt) {
-            load_vec<C, VW>(b[u], B, col[u] * ldb + n0);
-          } else {
-#pragma unroll
-            for (int k = 0; k < VW; ++k) b[u][k] = zero<C>();
-          }
-        }
+        gather_masked<VW, U>(b, B, ldb, col, n0, ok);
 #pragma unroll
         for (int u = 0; u < U; ++u)
           if (u0 + u < cnt)
@@ -362,12 +391,37 @@ __device__ __forceinline__ void spmm_staged_rows(const SpmmParams<V, I, P>& p, i
     C* sv = st_val[wv];
     for (int64_t w0 = wb; w0 < we; w0 += kMmStage) {
       const int64_t cnt = we - w0 < kMmStage ? we - w0 : kMmStage;
+      // all of the window's loads first, then its LDS stores: a load whose
+      // store sits in the same per-lane branch is waited for there, one slice
+      // of the window at a time.  Slices wholly past cnt are skipped by the
+      // wavefront (cnt is uniform); lanes past cnt re-read entry w0.
+      constexpr int kPer = kMmStage / kWave;
+      const int cntu = __builtin_amdgcn_readfirstlane((int)cnt);
+      int32_t lc[kPer];
+      V lv[kPer];
 #pragma unroll
-      for (int k = 0; k < kMmStage / kWave; ++k) {
+      for (int k = 0; k < kPer; ++k) {
+        if (k * kWave < cntu) {
+          const int i = k * kWave + lane;
+          const int64_t jj = w0 + (i < cntu ? i : 0);
+          lc[k] = (int32_t)p.idx[jj];
+          lv[k] = p.val[jj];
+        } else {
+          lc[k] = 0;
+          lv[k] = V(0);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        asm volatile("" : "+v"(lc[k]));
+        asm volatile("" : "+v"(lv[k]));
+      }
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
         const int i = k * kWave + lane;
-        if (i < cnt) {
-          sc[i] = (int32_t)p.idx[w0 + i];
-          sv[i] = load_as<C>(p.val, w0 + i);
+        if (i < cntu) {
+          sc[i] = lc[k];
+          sv[i] = (C)lv[k];
         }
       }
       wave_lds_sync();
@@ -384,16 +438,11 @@ __device__ __forceinline__ void spmm_staged_rows(const SpmmParams<V, I, P>& p, i
             col[u] = (int64_t)sc[jj - w0];
             a[u] = sv[jj - w0];
           }
+          unsigned ok = 0;
+#pragma unroll
+          for (int u = 0; u < U; ++u) ok |= (unsigned)(j + u < hi) << u;
           C b[U][VW];
-#pragma unroll
-          for (int u = 0; u < U; ++u) {
-            if (j + u < hi) {
-              load_vec<C, VW>(b[u], p.B, col[u] * p.ldb + n0);
-            } else {
-#pragma unroll
-              for (int k = 0; k < VW; ++k) b[u][k] = zero<C>();
-            }
-          }
+          gather_masked<VW, U>(b, p.B, p.ldb, col, n0, ok);
 #pragma unroll
           for (int u = 0; u < U; ++u)
             if (j + u < hi)
