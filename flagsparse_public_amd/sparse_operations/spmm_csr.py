@@ -266,8 +266,10 @@ def flagsparse_spmm_csr(data, indices, indptr, B, shape, block_n=None, block_nnz
     """C = A @ B (spmm_csr.py:2088-2157): row-major, non-transposed B and C.
 
     ``block_n`` / ``block_nnz`` / ``max_segments`` keep the reference's checks;
-    they do not change the HIP schedule.  Unlike the reference, no device
-    synchronisation happens unless ``return_time`` is set.
+    they do not change the HIP schedule.  Every call validates the matrix
+    with one device pass whose results are read on the host (one device
+    synchronisation, where the reference makes two); repeated products with
+    one matrix are cheaper through ``prepare_spmm_csr_opt``.
     """
     if block_n is not None and block_n <= 0:
         raise ValueError("block_n must be positive when provided")
