@@ -4,11 +4,11 @@
 //
 // Replaces the reference's Triton kernel (sddmm_csr.py:108-151: a program per
 // 128 entries, K in BLOCK_K steps, tl.sum reduction).  Here a group of G
-// lanes takes one entry: each lane reads VW consecutive elements of the x row
-// and of the y row per step (16-byte loads when the rows are aligned), keeps a
-// running sum, and the group reduces with DPP / swizzle exchanges.  G is the
-// smallest power of two covering K / VW (at most 64), so short K does not
-// leave lanes idle and a wavefront serves 64 / G entries.
+// lanes takes kSdE<G> entries: each lane reads VW consecutive elements of the x
+// row and of the y row per step (16-byte loads when the rows are aligned),
+// keeps a running sum per entry, and the group reduces with DPP / swizzle
+// exchanges.  G is the smallest power of two covering K / VW (at most 64), so
+// short K does not leave lanes idle.
 #include "fs_host.hpp"
 
 namespace fs {
@@ -37,27 +37,53 @@ __device__ __forceinline__ void sd_fma(A& acc, const T* xr, const T* yr, int64_t
   }
 }
 
+// Entries per lane group: the group's kSdE entries (kSdBlock / G apart, so
+// consecutive groups still read consecutive row ids / columns) have their row
+// and column loads, and then their x / y row loads, in flight together.
+// Measured on a 1M x 1M Poisson(16) pattern, fp32: one entry per group
+// 418 / 880 / 1969 us at K = 32 / 64 / 128; two 333 / 649 / 1472; four
+// 334 / 625 / 1324; eight at K = 128 (32-lane groups) 1304 (K = 8: 288, 290,
+// 293 us for one, two, four).
+template <int G> constexpr int kSdE = G >= 32 ? 8 : (G >= 16 ? 4 : 2);
+
 template <typename A, typename T, typename O, int G, int VW>
 __global__ __launch_bounds__(kSdBlock) void sddmm_kernel(
     const int32_t* __restrict__ row_ids, const int32_t* __restrict__ cols,
     const T* __restrict__ x, const T* __restrict__ y, const T* __restrict__ data,
     O* __restrict__ out, int64_t nnz, int64_t k_dim, double alpha, double beta) {
   using C = typename A::compute_t;
-  const int64_t p = (int64_t)blockIdx.x * (kSdBlock / G) + threadIdx.x / G;
+  constexpr int kGroups = kSdBlock / G;
+  constexpr int kE = kSdE<G>;
+  const int64_t p0 = (int64_t)blockIdx.x * kGroups * kE + threadIdx.x / G;
   const int g = threadIdx.x % G;
-  const bool live = p < nnz;  // the whole group agrees
-  A acc;
-  acc.init();
-  if (live) {
-    const T* xr = x + (int64_t)row_ids[p] * k_dim;
-    const T* yr = y + (int64_t)cols[p] * k_dim;
-    for (int64_t k = (int64_t)g * VW; k < k_dim; k += (int64_t)G * VW) sd_fma<A, T, VW>(acc, xr, yr, k);
+  const T* xr[kE];
+  const T* yr[kE];
+  bool live[kE];  // the whole group agrees
+#pragma unroll
+  for (int e = 0; e < kE; ++e) {
+    const int64_t p = p0 + (int64_t)e * kGroups;
+    live[e] = p < nnz;
+    // past nnz: the last entry again (nnz >= 1), computed and not written,
+    // so that no load sits under a per-lane branch
+    const int64_t q = live[e] ? p : nnz - 1;
+    xr[e] = x + (int64_t)row_ids[q] * k_dim;
+    yr[e] = y + (int64_t)cols[q] * k_dim;
   }
-  group_reduce<G>(acc);
-  if (live && g == 0) {
-    C v = acc.value() * (C)alpha;
-    if (data) v += (C)data[p] * (C)beta;
-    out[p] = (O)v;
+  A acc[kE];
+#pragma unroll
+  for (int e = 0; e < kE; ++e) acc[e].init();
+  for (int64_t k = (int64_t)g * VW; k < k_dim; k += (int64_t)G * VW)
+#pragma unroll
+    for (int e = 0; e < kE; ++e) sd_fma<A, T, VW>(acc[e], xr[e], yr[e], k);
+#pragma unroll
+  for (int e = 0; e < kE; ++e) {
+    group_reduce<G>(acc[e]);
+    if (live[e] && g == 0) {
+      const int64_t p = p0 + (int64_t)e * kGroups;
+      C v = acc[e].value() * (C)alpha;
+      if (data) v += (C)data[p] * (C)beta;
+      out[p] = (O)v;
+    }
   }
 }
 
@@ -65,7 +91,8 @@ template <typename A, typename T, typename O, int G>
 void sddmm_launch_g(const int32_t* rid, const int32_t* cols, const T* x, const T* y,
                     const T* data, O* out, int64_t nnz, int64_t k, double alpha, double beta,
                     int vw, hipStream_t st) {
-  const int64_t blocks = (nnz + kSdBlock / G - 1) / (kSdBlock / G);
+  const int64_t per_block = (int64_t)(kSdBlock / G) * kSdE<G>;
+  const int64_t blocks = (nnz + per_block - 1) / per_block;
   const dim3 grid((unsigned)blocks), block(kSdBlock);
   constexpr int kVec = sizeof(T) == 4 ? 4 : 2;
   if (vw == kVec)
