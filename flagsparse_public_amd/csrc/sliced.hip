@@ -77,64 +77,89 @@ __global__ __launch_bounds__(kCombineRows) void sliced_combine_kernel(
 // the slices.
 constexpr int kMaskWords = kCombineRows / 32;
 
-template <typename Z, typename V>
+// kRB consecutive 256-row blocks per workgroup (thread t owns row t of each):
+// the blocks' mask words and bounds are staged together and each thread has
+// kRB * kGroup partial loads in flight.
+template <typename Z, typename V, int kRB>
 __global__ __launch_bounds__(kCombineRows) void sliced_combine_mask_kernel(
     const Z* __restrict__ z, const uint32_t* __restrict__ row_mask,
     const int64_t* __restrict__ bounds, int32_t n_slices, int64_t n_rows, int64_t n_blocks,
     int64_t n_compact, V* __restrict__ y, double alpha, double beta) {
   using C = typename Native<Z>::T;
   using S = typename TileSum<C>::T;
-  __shared__ uint32_t words[kGroup][kMaskWords];
-  __shared__ int32_t before[kGroup][kMaskWords];  // set bits in the words before j
-  __shared__ int64_t base[kGroup];
-  const int64_t b = blockIdx.x;
+  static_assert(kRB * kGroup * kMaskWords <= 128 && kRB * kGroup <= 128, "staging threads");
+  __shared__ uint32_t words[kRB][kGroup][kMaskWords];
+  __shared__ int32_t before[kRB][kGroup][kMaskWords];  // set bits in the words before j
+  __shared__ int64_t base[kRB][kGroup];
+  const int64_t b0 = (int64_t)blockIdx.x * kRB;
   const int tid = threadIdx.x;
-  const int64_t row = b * kCombineRows + tid;
-  S acc = zero<S>();
+  S acc[kRB];
+#pragma unroll
+  for (int r = 0; r < kRB; ++r) acc[r] = zero<S>();
   if (n_compact == 0) {  // uniform: no partials to read
-    if (row < n_rows) write_out(y, row, acc, alpha, beta);
+#pragma unroll
+    for (int r = 0; r < kRB; ++r) {
+      const int64_t row = (b0 + r) * kCombineRows + tid;
+      if (row < n_rows) write_out(y, row, acc[r], alpha, beta);
+    }
     return;
   }
   const int w = tid / 32, bit = tid % 32;
   const uint32_t below = bit == 0 ? 0u : (0xffffffffu >> (32 - bit));
   for (int k0 = 0; k0 < n_slices; k0 += kGroup) {
     if (k0 > 0) __syncthreads();  // the previous group's words are consumed
-    if (tid < kGroup * kMaskWords) {
-      const int u = tid / kMaskWords, j = tid % kMaskWords;
-      words[u][j] = k0 + u < n_slices ? row_mask[((int64_t)(k0 + u) * n_blocks + b) * kMaskWords + j]
-                                      : 0u;
-    } else if (tid >= 128 && tid < 128 + kGroup) {
-      const int u = tid - 128;
-      base[u] = k0 + u < n_slices ? bounds[(int64_t)(k0 + u) * (n_blocks + 1) + b] : 0;
+    if (tid < kRB * kGroup * kMaskWords) {
+      const int r = tid / (kGroup * kMaskWords), u = (tid / kMaskWords) % kGroup;
+      const int j = tid % kMaskWords;
+      const int64_t b = b0 + r;
+      words[r][u][j] = k0 + u < n_slices && b < n_blocks
+                           ? row_mask[((int64_t)(k0 + u) * n_blocks + b) * kMaskWords + j]
+                           : 0u;
+    } else if (tid >= 128 && tid < 128 + kRB * kGroup) {
+      const int r = (tid - 128) / kGroup, u = (tid - 128) % kGroup;
+      const int64_t b = b0 + r;
+      base[r][u] = k0 + u < n_slices && b < n_blocks ? bounds[(int64_t)(k0 + u) * (n_blocks + 1) + b]
+                                                     : 0;
     }
     __syncthreads();
-    if (tid < kGroup) {
+    if (tid < kRB * kGroup) {
+      const int r = tid / kGroup, u = tid % kGroup;
       int c = 0;
 #pragma unroll
       for (int j = 0; j < kMaskWords; ++j) {
-        before[tid][j] = c;
-        c += __popc(words[tid][j]);
+        before[r][u][j] = c;
+        c += __popc(words[r][u][j]);
       }
     }
     __syncthreads();
-    C zv[kGroup];
-    unsigned present = 0;  // slices of this group holding a partial of the row
+    C zv[kRB][kGroup];
+    unsigned present[kRB];  // slices of this group holding a partial of the row
 #pragma unroll
-    for (int u = 0; u < kGroup; ++u) {
-      const uint32_t mine = words[u][w];
-      const int64_t i = base[u] + before[u][w] + __popc(mine & below);
-      const bool has = ((mine >> bit) & 1u) && i < n_compact;  // a bad mask reads nothing
-      // every lane loads (absent rows read partial 0) so that the kGroup
-      // loads stay unconditional and are all in flight before the first add;
-      // a load under a per-lane branch is waited for inside the branch
-      zv[u] = load_as<C>(z, has ? i : 0);
-      present |= (unsigned)has << u;
+    for (int r = 0; r < kRB; ++r) {
+      present[r] = 0;
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u) {
+        const uint32_t mine = words[r][u][w];
+        const int64_t i = base[r][u] + before[r][u][w] + __popc(mine & below);
+        const bool has = ((mine >> bit) & 1u) && i < n_compact;  // a bad mask reads nothing
+        // every lane loads (absent rows read partial 0) so that the loads
+        // stay unconditional and are all in flight before the first add; a
+        // load under a per-lane branch is waited for inside the branch
+        zv[r][u] = load_as<C>(z, has ? i : 0);
+        present[r] |= (unsigned)has << u;
+      }
     }
 #pragma unroll
-    for (int u = 0; u < kGroup; ++u)
-      if ((present >> u) & 1u) acc = add(acc, widen(zv[u]));
+    for (int r = 0; r < kRB; ++r)
+#pragma unroll
+      for (int u = 0; u < kGroup; ++u)
+        if ((present[r] >> u) & 1u) acc[r] = add(acc[r], widen(zv[r][u]));
   }
-  if (row < n_rows) write_out(y, row, acc, alpha, beta);
+#pragma unroll
+  for (int r = 0; r < kRB; ++r) {
+    const int64_t row = (b0 + r) * kCombineRows + tid;
+    if (row < n_rows) write_out(y, row, acc[r], alpha, beta);
+  }
 }
 
 template <typename Z, typename V>
@@ -142,14 +167,18 @@ int launch_combine(const fs_slices* sl, const void* z, void* y, double alpha, do
                    hipStream_t st) {
   const int64_t nb = (sl->n_rows + kCombineRows - 1) / kCombineRows;
   if (nb == 0) return FS_OK;
-  if (sl->row_mask)
-    sliced_combine_mask_kernel<Z, V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
+  if (sl->row_mask) {
+    // two row blocks per workgroup where the partials are at most 8 bytes
+    constexpr int kRB = sizeof(typename Native<Z>::T) <= 8 ? 2 : 1;
+    sliced_combine_mask_kernel<Z, V, kRB><<<dim3((unsigned)((nb + kRB - 1) / kRB)),
+                                            dim3(kCombineRows), 0, st>>>(
         static_cast<const Z*>(z), sl->row_mask, sl->bounds, sl->n_slices, sl->n_rows, nb,
         sl->n_compact, static_cast<V*>(y), alpha, beta);
-  else
+  } else {
     sliced_combine_kernel<Z, V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
         static_cast<const Z*>(z), sl->compact_row, sl->bounds, sl->n_slices, sl->n_rows, nb,
         static_cast<V*>(y), alpha, beta);
+  }
   return check_launch("fs_sliced_combine");
 }
 
