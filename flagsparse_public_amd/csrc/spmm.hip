@@ -235,6 +235,10 @@ __device__ __forceinline__ void mm_accumulate(A (&acc)[VW], const I* __restrict_
 // stream), parked in the group's LDS slots and read back U at a time.  Lanes
 // past the last dense column (colok false) still load their share.  The
 // products are added in entry order, as in mm_accumulate.
+// F fills per round: each lane loads F entries (F * G slots per group), so a
+// row of up to F * G entries needs one fill round trip instead of F.
+template <int G> constexpr int kMmFill = G <= 8 ? 2 : 1;
+
 template <int VW, int U, int G, typename A, typename V>
 __device__ __forceinline__ void mm_accumulate_group(A (&acc)[VW], int32_t* slot_col,
                                                     typename A::compute_t* slot_val,
@@ -244,15 +248,35 @@ __device__ __forceinline__ void mm_accumulate_group(A (&acc)[VW], int32_t* slot_
                                                     int64_t n0, bool colok, int64_t s,
                                                     int64_t e, int g) {
   using C = typename A::compute_t;
+  constexpr int F = kMmFill<G>;
   static_assert(G % U == 0, "batches of U entries tile the group's G slots");
-  for (int64_t j0 = s; j0 < e; j0 += G) {
-    const int64_t jj = j0 + g;
-    slot_col[g] = jj < e ? idx[jj] : 0;
-    slot_val[g] = jj < e ? load_as<C>(val, jj) : zero<C>();
+  for (int64_t j0 = s; j0 < e; j0 += G * F) {
+    if constexpr (F == 1) {
+      const int64_t jj = j0 + g;
+      slot_col[g] = jj < e ? idx[jj] : 0;
+      slot_val[g] = jj < e ? load_as<C>(val, jj) : zero<C>();
+    } else {
+      // unconditional loads (past e: entry j0 again), then the LDS stores,
+      // so the F loads of a lane are in flight together
+      int32_t fc[F];
+      V fv[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        const int64_t jj = j0 + f * G + g;
+        const int64_t q = jj < e ? jj : j0;
+        fc[f] = idx[q];
+        fv[f] = val[q];
+      }
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        slot_col[f * G + g] = fc[f];
+        slot_val[f * G + g] = (C)fv[f];
+      }
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    const int cnt = e - j0 < G ? (int)(e - j0) : G;
+    const int cnt = e - j0 < G * F ? (int)(e - j0) : G * F;
     if (colok) {
       for (int u0 = 0; u0 < cnt; u0 += U) {
         int64_t col[U];
@@ -482,9 +506,10 @@ __global__ __launch_bounds__(kMmBlock) void spmm_csr_kernel(SpmmParams<V, I, P> 
   for (int k = 0; k < VW; ++k) acc[k].init();
   if constexpr (kGroupLoads<V, I, G, U>) {
     using C = typename A::compute_t;
-    __shared__ int32_t slot_col[kMmBlock];
-    __shared__ C slot_val[kMmBlock];
-    const int first = threadIdx.x - g;  // the group's slots
+    constexpr int F = kMmFill<G>;
+    __shared__ int32_t slot_col[kMmBlock * F];
+    __shared__ C slot_val[kMmBlock * F];
+    const int first = (threadIdx.x - g) * F;  // the group's F * G slots
     const bool colok = n0 < p.n_dense;
     mm_accumulate_group<VW, U, G>(acc, slot_col + first, slot_val + first, p.idx, p.val, p.B,
                                   p.ldb, n0, colok, rs, re, g);
