@@ -216,19 +216,21 @@ def test_spmv_entry_tiles_all_dtypes(orc, dtype, index_dtype):
             assert (np.abs(got - want) / np.maximum(scale_, 1e-300)).max() <= bound
 
 
+@pytest.mark.parametrize("n_rows", [30001, 29900], ids=["118blocks", "117blocks"])
 @pytest.mark.parametrize("use_opt", [False, True], ids=["sums", "native"])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64, torch.complex64],
                          ids=["f32", "f64", "c64"])
-def test_sliced_combine_row_masks_equal_row_ids(dtype, use_opt):
+def test_sliced_combine_row_masks_equal_row_ids(dtype, use_opt, n_rows):
     """The combine's two ways to find a row's partials -- presence masks with
     popcounts, or the compact-row ids -- give the same bits (fp64 row sums of
     the default fp32 path, native partials otherwise), including 24 slices
-    (three groups of 8), a row count that ends inside a 32-row mask word, and
+    (three groups of 8), a row count that ends inside a 32-row mask word, an
+    odd number of 256-row blocks (the mask kernel takes two per workgroup) and
     alpha / beta."""
     from flagsparse_public_amd import _lib
     npd = {torch.float32: np.float32, torch.float64: np.float64,
            torch.complex64: np.complex64}[dtype]
-    ip, cols, vals = _powerlaw(30001, 5, 20000, np.float64)
+    ip, cols, vals = _powerlaw(n_rows, 5, 20000, np.float64)
     n = ip.size - 1
     assert n % 32 != 0
     rng = np.random.default_rng(8)
