@@ -37,6 +37,32 @@ def test_sddmm_matches_oracle(orc, dtype, npd, tol, k, with_data):
     assert (np.abs(got - want) / np.maximum(cond, 1e-300)).max() <= tol
 
 
+@pytest.mark.parametrize("nnz", [1, 5, 63, 65, 129, 1000])
+@pytest.mark.parametrize("k", [4, 64, 300])
+def test_sddmm_small_patterns_partial_blocks(orc, nnz, k):
+    """Entry counts that leave the last workgroup partly empty (a lane group
+    takes 2 / 4 / 8 entries for K = 4 / 64 / 300): every entry against the
+    oracle, and nothing written past the last entry of the output."""
+    rng = np.random.default_rng(nnz * 1000 + k)
+    n_rows, n_cols = 13, 11
+    rows = np.sort(rng.integers(0, n_rows, nnz))
+    cols = rng.integers(0, n_cols, nnz).astype(np.int32)
+    ip = np.zeros(n_rows + 1, np.int64)
+    np.add.at(ip, rows + 1, 1)
+    ip = np.cumsum(ip)
+    x = rng.uniform(-1, 1, (n_rows, k)).astype(np.float32)
+    y = rng.uniform(-1, 1, (n_cols, k)).astype(np.float32)
+    want, cond = orc.sddmm(ip, cols, x, y, None, 1.0, 0.0)
+    buf = torch.full((nnz + 512,), 7.0, device=DEV)
+    out = flagsparse_sddmm_csr(indices=torch.from_numpy(cols).to(DEV),
+                               indptr=torch.from_numpy(ip).to(DEV), x=torch.from_numpy(x).to(DEV),
+                               y=torch.from_numpy(y).to(DEV), shape=(n_rows, n_cols),
+                               out=buf[:nnz])
+    got = out.cpu().numpy().astype(np.float64)
+    assert (np.abs(got - want) / np.maximum(cond, 1e-300)).max() <= 2e-6
+    assert torch.all(buf[nnz:] == 7.0)
+
+
 def test_sddmm_variants_meta_and_prepared(orc):
     ip, cols, _ = synthetic.poisson_csr(2000, 10.0, 3)
     n = ip.size - 1
