@@ -301,6 +301,38 @@ __device__ __forceinline__ void spmv_tile_role(const SpmvParams<VS, V, I, P>& p,
   }
   // products of the tile's entries, U per thread in flight
   for (int64_t base = e0 + threadIdx.x; base < e1; base += (int64_t)kBlock * U) {
+    if constexpr (HasRaw<VS>::value && std::is_arithmetic<V>::value) {
+      // raw loads kept opaque until all are issued, then converted (a
+      // conversion inside a masked load's branch brings a wait with it)
+      using R = typename VS::raw_t;
+      I ci[U];
+      R ar[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t j = base + (int64_t)u * kBlock;
+        ci[u] = j < e1 ? p.idx[j] : I(0);
+        ar[u] = j < e1 ? p.vals.raw(j) : R(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) asm volatile("" : "+v"(ci[u]));
+      V xr[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t j = base + (int64_t)u * kBlock;
+        xr[u] = j < e1 ? p.x[(int64_t)ci[u]] : V(0);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        asm volatile("" : "+v"(ar[u]));
+        asm volatile("" : "+v"(xr[u]));
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t j = base + (int64_t)u * kBlock;
+        if (j < e1) prod[j - e0] = widen(mulc(p.vals.template conv<C>(ar[u]), (C)xr[u]));
+      }
+      continue;
+    }
     int64_t col[U];
     C a[U];
 #pragma unroll
