@@ -500,7 +500,7 @@ def main(argv=None):
                          "tile_nnz": sch.tile_nnz, "x_slice_cols": sch.x_slice_cols,
                          "n_slices": sliced.n_slices if sliced is not None else 1,
                          "compact_rows": nc,
-                         "chosen_by": prepared.opt_buckets[0].get("chosen_by")},
+                         "chosen_by": prepared.schedule_record["chosen_by"]},
             "gflops": round(2.0 * nnz / (per_launch_ms * 1e-3) / 1e9, 2),
             "compulsory_bytes": (n + 1) * 4 + nnz * (4 + s_v) + n * s_v + n * s_v,
         },

@@ -77,6 +77,7 @@ class BatchedCsr:
         self.entries = (FsBatchEntry * len(matrices))()
         self.keep = []  # tensors referenced by device records
         self.y, self.C = [], []
+        self.group_lanes = []  # SpMV lanes per row of each matrix
         items = []
         bytes_total = 0
         for i, m in enumerate(matrices):
@@ -111,6 +112,7 @@ class BatchedCsr:
             e.B, e.C = _lib.ptr(B), _lib.ptr(C)
             e.n_dense, e.ldb, e.ldc = n_dense, n_dense, n_dense
             e.group_lanes = int(s)
+            self.group_lanes.append(int(s))
             self.keep.extend([data, indices, indptr, x, B])
             self.y.append(y)
             self.C.append(C)

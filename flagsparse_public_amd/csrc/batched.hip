@@ -323,6 +323,10 @@ extern "C" int fs_batch_run(const void* packed_dev, int64_t n_matrices, const fs
   if (!packed_dev || !items || !queue) return fail(FS_ERR_VALUE, "fs_batch_run: null pointer");
   if (grid <= 0) return fail(FS_ERR_VALUE, "fs_batch_run: grid must be positive");
   if (long_row < 1) return fail(FS_ERR_VALUE, "fs_batch_run: long_row must be >= 1");
+  // the work queue head is an int32 that every block advances once past the
+  // last item; it must not wrap
+  if (n_items + (int64_t)grid > (int64_t)INT32_MAX)
+    return fail(FS_ERR_UNSUPPORTED, "fs_batch_run: n_items + grid must stay below 2^31 - 1");
   batched_kernel<<<grid, kBatBlock, 0, as_stream(stream)>>>(static_cast<const BatchMatrix*>(packed_dev),
                                                            items, n_items, long_row, alpha, beta, queue);
   return check_launch("fs_batch_run");

@@ -69,6 +69,9 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
   if (P->tile_nnz < 0 || P->n_tiles < 0) return fail(FS_ERR_VALUE, "%s: negative tile fields", who);
   if (P->tile_mode != FS_TILE_ROWS && P->tile_mode != FS_TILE_ENTRIES)
     return fail(FS_ERR_VALUE, "%s: tile_mode must be FS_TILE_ROWS or FS_TILE_ENTRIES", who);
+  // only the entry-tile role stores raw row sums; every other role writes y
+  if (P->row_sums && !(P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES))
+    return fail(FS_ERR_VALUE, "%s: row_sums are offered for entry tiles only", who);
   if (P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES) {
     if (n_dense != 1) return fail(FS_ERR_VALUE, "%s: entry tiles are for SpMV", who);
     if (P->n_long != 0) return fail(FS_ERR_VALUE, "%s: entry tiles take no long-row lists", who);

@@ -229,6 +229,15 @@ def _plan_for(cache, kernel_indptr, n_rows, dtype, accumulate, schedule, row_len
     return p
 
 
+def _spmm_target(out, beta):
+    """Where the kernel writes C: ``out`` itself when it is contiguous; for a
+    strided ``out``, a contiguous copy of it when beta != 0 (the kernel reads
+    C), else a new tensor; the caller copies the result back into ``out``."""
+    if out is None or out.is_contiguous():
+        return out
+    return out.contiguous() if float(beta) != 0.0 else None
+
+
 def _launch_spmm(csr, plan, B, n_rows, n_dense, dtype, device, out=None, alpha=1.0, beta=0.0,
                  col_tile=0):
     C = out if out is not None else torch.empty((n_rows, n_dense), dtype=dtype, device=device)
@@ -290,14 +299,14 @@ def flagsparse_spmm_csr(data, indices, indptr, B, shape, block_n=None, block_nnz
             raise ValueError("out shape/dtype must match result")
     if float(beta) != 0.0 and out is None:
         raise ValueError("beta != 0 requires out (the C that is scaled and accumulated)")
-    direct = out is not None and out.is_contiguous()
+    target = _spmm_target(out, beta)
     t0 = None
     if return_time:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
     C = _hip_spmm_csr_impl(data, kernel_indices, kernel_indptr, B, n_rows, n_dense_cols,
                            block_n=block_n, block_nnz=block_nnz, row_lengths=row_lengths,
-                           n_cols=n_cols, out=out if direct else None, alpha=alpha, beta=beta)
+                           n_cols=n_cols, out=target, alpha=alpha, beta=beta)
     elapsed_ms = None
     if return_time:
         torch.cuda.synchronize()
@@ -321,7 +330,7 @@ class PreparedCsrSpmmOpt:
         "data", "kernel_indices", "kernel_indptr", "shape", "n_rows", "n_cols", "row_lengths",
         "max_row_nnz", "row_buckets", "supports_opt", "long_part_rows", "long_part_starts",
         "long_part_ends", "long_row_ids", "long_row_part_ptr", "long_row_fallback_only",
-        "schedule", "library_schedule", "_plans", "_csr",
+        "schedule", "library_schedule", "_plans", "_csr", "schedule_record",
     )
 
     def __init__(self, data, kernel_indices, kernel_indptr, shape, n_rows, n_cols, row_lengths,
@@ -346,6 +355,7 @@ class PreparedCsrSpmmOpt:
         self.long_row_fallback_only = bool(long_row_fallback_only)
         self.schedule = schedule
         self.library_schedule = bool(library_schedule)  # chosen by the library, not the caller
+        self.schedule_record = None  # how the library chose `schedule`
         self._plans = {}
         self._csr = None
 
@@ -433,16 +443,18 @@ def prepare_spmm_csr_opt(data, indices, indptr, shape, schedule=None):
     (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
      max_row_nnz) = _prepare_spmm_csr_matrix(data, indices, indptr, shape, _stats_out=stats_out)
     library = not isinstance(schedule, Schedule)
-    schedule, _ = choose_schedule("spmm64", schedule, data, n_rows, n_cols, stats_out[0])
+    schedule, how = choose_schedule("spmm64", schedule, data, n_rows, n_cols, stats_out[0])
     row_buckets, long_rows = _build_spmm_opt_buckets(row_lengths)
     part_rows, part_starts, part_ends, part_ptr = _build_spmm_opt_split_metadata(
         kernel_indptr, long_rows, 256)
-    return PreparedCsrSpmmOpt(
+    prepared = PreparedCsrSpmmOpt(
         data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
         n_rows=n_rows, n_cols=n_cols, row_lengths=row_lengths, max_row_nnz=max_row_nnz,
         row_buckets=row_buckets, long_part_rows=part_rows, long_part_starts=part_starts,
         long_part_ends=part_ends, long_row_ids=long_rows, long_row_part_ptr=part_ptr,
         long_row_fallback_only=False, schedule=schedule, library_schedule=library)
+    prepared.schedule_record = {"schedule": schedule, "chosen_by": how}
+    return prepared
 
 
 def _check_opt_B(prepared, B):
@@ -646,13 +658,12 @@ def flagsparse_spmm_csr_opt(data=None, indices=None, indptr=None, B=None, shape=
             raise ValueError("out shape/dtype must match result")
     if float(beta) != 0.0 and out is None:
         raise ValueError("beta != 0 requires out (the C that is scaled and accumulated)")
-    direct = out is not None and out.is_contiguous()
+    target = _spmm_target(out, beta)
     t0 = None
     if return_time:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-    C, _ = _triton_spmm_csr_impl_opt_prepared(prepared, B, out=out if direct else None,
-                                              alpha=alpha, beta=beta)
+    C, _ = _triton_spmm_csr_impl_opt_prepared(prepared, B, out=target, alpha=alpha, beta=beta)
     elapsed_ms = None
     if return_time:
         torch.cuda.synchronize()

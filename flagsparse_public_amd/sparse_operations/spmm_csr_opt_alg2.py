@@ -23,6 +23,7 @@ from .spmm_csr import (  # noqa: F401  (re-exported like the reference)
     _plan_for,
     _prepare_spmm_csr_matrix,
     _spmm_coo_reference_tolerance,
+    _spmm_target,
     _spmm_validation_metrics,
     flagsparse_spmm_csr,
     flagsparse_spmm_csr_opt,
@@ -299,14 +300,13 @@ def flagsparse_spmm_csr_opt_alg2(data=None, indices=None, indptr=None, B=None, s
     if float(beta) != 0.0 and out is None:
         raise ValueError("beta != 0 requires out (the C that is scaled and accumulated)")
     B = B.contiguous()
-    direct = out is not None and out.is_contiguous()
+    target = _spmm_target(out, beta)
     t0 = None
     if return_time:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
     C, meta = _triton_spmm_csr_impl_opt_alg2_prepared(prepared, B, return_meta=return_meta,
-                                                      out=out if direct else None, alpha=alpha,
-                                                      beta=beta)
+                                                      out=target, alpha=alpha, beta=beta)
     elapsed_ms = None
     if return_time:
         torch.cuda.synchronize()

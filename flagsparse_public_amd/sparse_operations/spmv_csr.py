@@ -78,17 +78,23 @@ def _spmv_op_transposes(op):
 class PreparedCsrSpmv:
     """Cached CSR metadata for repeated SpMV calls (spmv_csr.py:55-122).
 
-    Holds the same fields as the reference; ``opt_buckets`` describes the
-    row-split schedule.  Device plans are built on first use and cached per
-    accumulation mode, so a directly constructed instance works too.
+    Holds the same fields as the reference.  ``opt_buckets`` holds the
+    reference's row-length bucket lists (``_build_spmv_opt_buckets``); when
+    the constructor gets ``opt_buckets=None`` they are built on first read,
+    so calls that never read them pay no host synchronisation for them.
+    ``schedule`` is the HIP schedule that actually runs and
+    ``schedule_record`` says how it was chosen.  Device plans are built on
+    first use and cached per accumulation mode, so a directly constructed
+    instance works too.
     """
 
     __slots__ = (
         "data", "kernel_indices", "kernel_indptr", "shape", "n_rows", "n_cols",
-        "block_nnz", "max_segments", "max_row_nnz", "opt_buckets", "supports_opt",
+        "block_nnz", "max_segments", "max_row_nnz", "_opt_buckets", "supports_opt",
         "transpose", "op", "index_fallback_policy", "index_fallback_applied",
         "index_fallback_reason", "_baseline_compute_dtype", "_baseline_data",
-        "schedule", "_plans", "_csr", "row_lengths",
+        "schedule", "_plans", "_csr", "row_lengths", "schedule_record",
+        "_bucket_max_segments",
     )
 
     def __init__(self, data, kernel_indices, kernel_indptr, shape, n_rows, n_cols, block_nnz,
@@ -104,7 +110,9 @@ class PreparedCsrSpmv:
         self.block_nnz = block_nnz
         self.max_segments = max_segments
         self.max_row_nnz = max_row_nnz
-        self.opt_buckets = opt_buckets
+        self._opt_buckets = opt_buckets
+        self._bucket_max_segments = None
+        self.schedule_record = None
         self.supports_opt = (data.dtype in (torch.float32, torch.float64)
                              and kernel_indices.dtype == torch.int32)
         self.op = _normalize_spmv_op(op, transpose=transpose)
@@ -123,6 +131,24 @@ class PreparedCsrSpmv:
         self.row_lengths = row_lengths
         self._plans = {}
         self._csr = None
+
+    @property
+    def opt_buckets(self):
+        """Row lists per length bucket, as spmv_csr.py:345-391 builds them."""
+        if self._opt_buckets is None:
+            lens = self.row_lengths
+            if lens is None:
+                lens = (self.kernel_indptr[1:] - self.kernel_indptr[:-1]).to(torch.int64)
+            row_index_dtype = torch.int32 if self.n_rows <= _INDEX_LIMIT_INT32 else torch.int64
+            self._opt_buckets = _build_spmv_opt_buckets(
+                lens, max_row_nnz=self.max_row_nnz, row_index_dtype=row_index_dtype,
+                max_segments=self._bucket_max_segments,
+                fp64=self.data.dtype == torch.float64)
+        return self._opt_buckets
+
+    @opt_buckets.setter
+    def opt_buckets(self, value):
+        self._opt_buckets = value
 
     # -- device plumbing (not part of the reference surface) --------------
     def csr_struct(self):
@@ -192,13 +218,10 @@ def _build_spmv_opt_buckets(row_lengths, max_row_nnz, row_index_dtype, max_segme
     return buckets
 
 
-def _schedule_buckets(plan_schedule, n_rows, long_rows):
-    """Introspection record of the row-split schedule (stored as opt_buckets)."""
-    return [
-        {"kind": "short", "group_lanes": plan_schedule.group_lanes,
-         "short_max": plan_schedule.short_max, "n_rows": n_rows},
-        {"kind": "chunk", "chunk_nnz": plan_schedule.chunk_nnz, "rows": long_rows},
-    ]
+def _schedule_record(sch, how):
+    """How the HIP schedule of a prepared matrix was chosen (not a reference
+    field; the reference's ``opt_buckets`` keep their meaning)."""
+    return {"schedule": sch, "chosen_by": how}
 
 
 def _normalize_spmv_index_fallback_policy(index_fallback_policy):
@@ -337,13 +360,13 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
     prepared = PreparedCsrSpmv(
         data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
         n_rows=n_rows, n_cols=n_cols, block_nnz=block_nnz_use, max_segments=max_segments_use,
-        max_row_nnz=max_row_nnz, opt_buckets=[], transpose=transpose, op=op_code,
+        max_row_nnz=max_row_nnz, opt_buckets=None, transpose=transpose, op=op_code,
         index_fallback_policy=index_fallback_policy, schedule=sch, row_lengths=row_lengths)
+    prepared._bucket_max_segments = max_segments
     if how == "exhaustive":
         sch = exhaustive_spmv_schedule(prepared)
         prepared.schedule = sch
-    prepared.opt_buckets = _schedule_buckets(sch, n_rows, None)
-    prepared.opt_buckets[0]["chosen_by"] = how
+    prepared.schedule_record = _schedule_record(sch, how)
     return prepared
 
 
@@ -431,29 +454,44 @@ def _spmv_prepared_with_int32_indices(prepared, reason):
         data=prepared.data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr,
         shape=prepared.shape, n_rows=prepared.n_rows, n_cols=prepared.n_cols,
         block_nnz=prepared.block_nnz, max_segments=prepared.max_segments,
-        max_row_nnz=max_row_nnz, opt_buckets=[], transpose=prepared.transpose, op=prepared.op,
+        max_row_nnz=max_row_nnz, opt_buckets=None, transpose=prepared.transpose, op=prepared.op,
         index_fallback_policy=prepared.index_fallback_policy, index_fallback_applied=True,
         index_fallback_reason=str(reason), schedule=prepared.schedule, row_lengths=row_lengths)
-    fb.opt_buckets = list(prepared.opt_buckets)
+    fb._bucket_max_segments = prepared.max_segments  # as spmv_csr.py:751-757 rebuilds them
+    fb.schedule_record = prepared.schedule_record
     return fb
 
 
-def _run_spmv_prepared(prepared, x, use_opt=False):
+def _launch_kwargs(out, alpha, beta):
+    """Keyword arguments for a launcher hook: none with the reference's
+    defaults, so hooks patched with the reference's two-argument signature
+    keep working."""
+    if out is None and float(alpha) == 1.0 and float(beta) == 0.0:
+        return {}
+    return {"out": out, "alpha": alpha, "beta": beta}
+
+
+def _run_spmv_prepared(prepared, x, use_opt=False, out=None, alpha=1.0, beta=0.0):
+    """One launch (spmv_csr.py:777-780); ``out`` / ``alpha`` / ``beta`` are
+    the trailing extension (y = alpha A x + beta out, written into ``out``)."""
+    kw = _launch_kwargs(out, alpha, beta)
     if use_opt and prepared.supports_opt:
-        return _triton_spmv_csr_impl_opt_prepared(prepared, x)
-    return _triton_spmv_csr_impl_prepared(prepared, x)
+        return _triton_spmv_csr_impl_opt_prepared(prepared, x, **kw)
+    return _triton_spmv_csr_impl_prepared(prepared, x, **kw)
 
 
-def _run_spmv_prepared_with_fallback(prepared, x, use_opt=False):
+def _run_spmv_prepared_with_fallback(prepared, x, use_opt=False, out=None, alpha=1.0, beta=0.0):
     """Retry with int32 indices after a failed int64 launch under policy
-    'auto' (spmv_csr.py:783-793)."""
+    'auto' (spmv_csr.py:783-793).  Launch failures are raised before any
+    kernel runs, so a retry with beta != 0 still reads the caller's y."""
+    kw = _launch_kwargs(out, alpha, beta)
     try:
-        return _run_spmv_prepared(prepared, x, use_opt=use_opt)
+        return _run_spmv_prepared(prepared, x, use_opt=use_opt, **kw)
     except Exception as exc:
         if prepared.index_fallback_policy != "auto" or not _spmv_uses_int64_indices(prepared):
             raise
         fb = _spmv_prepared_with_int32_indices(prepared, exc)
-        return _run_spmv_prepared(fb, x, use_opt=use_opt)
+        return _run_spmv_prepared(fb, x, use_opt=use_opt, **kw)
 
 
 def flagsparse_spmv_csr(data=None, indices=None, indptr=None, x=None, shape=None,
@@ -494,22 +532,18 @@ def flagsparse_spmv_csr(data=None, indices=None, indptr=None, x=None, shape=None
     if float(beta) != 0.0 and out is None:
         raise ValueError("beta != 0 requires out (the y that is scaled and accumulated)")
     # with a contiguous `out` the kernels write it directly (the reference
-    # computes a new y and copies it into `out`: same values, one pass fewer)
-    direct = out is not None and out.is_contiguous()
+    # computes a new y and copies it into `out`: same values, one pass fewer);
+    # a strided `out` is staged through a contiguous copy when beta reads it
+    if out is None or out.is_contiguous():
+        target = out
+    else:
+        target = out.contiguous() if float(beta) != 0.0 else None
     t0 = None
     if return_time:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-    if direct:
-        try:
-            y = _direct_spmv(prepared, x, out, use_opt, alpha, beta)
-        except Exception as exc:
-            if prepared.index_fallback_policy != "auto" or not _spmv_uses_int64_indices(prepared):
-                raise
-            fb = _spmv_prepared_with_int32_indices(prepared, exc)
-            y = _direct_spmv(fb, x, out, use_opt, alpha, beta)
-    else:
-        y = _run_spmv_prepared_with_fallback(prepared, x, use_opt=use_opt)
+    y = _run_spmv_prepared_with_fallback(prepared, x, use_opt=use_opt, out=target, alpha=alpha,
+                                         beta=beta)
     elapsed_ms = None
     if return_time:
         torch.cuda.synchronize()
@@ -520,12 +554,6 @@ def flagsparse_spmv_csr(data=None, indices=None, indptr=None, x=None, shape=None
     if return_time:
         return y, elapsed_ms
     return y
-
-
-def _direct_spmv(prepared, x, out, use_opt, alpha, beta):
-    if use_opt and prepared.supports_opt:
-        return _hip_spmv_csr_impl_opt_prepared(prepared, x, out=out, alpha=alpha, beta=beta)
-    return _hip_spmv_csr_impl_prepared(prepared, x, out=out, alpha=alpha, beta=beta)
 
 
 # ---------------------------------------------------------------- COO -> CSR
@@ -541,7 +569,9 @@ def _coo_is_sorted_lex(row_i64, col_i64, n_cols):
 
 def coo_to_csr_for_spmv(data, row, col, shape, assume_sorted=False):
     """COO -> (data, int32 columns, int64 indptr) sorted by (row, col)
-    (spmv_csr.py:885-912)."""
+    (spmv_csr.py:885-912).  The sort is stable (the reference's argsort is
+    not), so repeated (row, col) pairs keep their input order and are summed
+    in that order: the result is reproducible bit for bit."""
     n_rows, n_cols = int(shape[0]), int(shape[1])
     row64, col64 = row.to(torch.int64), col.to(torch.int64)
     if row64.numel() == 0:
@@ -550,7 +580,7 @@ def coo_to_csr_for_spmv(data, row, col, shape, assume_sorted=False):
     if assume_sorted or _coo_is_sorted_lex(row64, col64, n_cols):
         row_s, col_s, data_s = row64, col64, data
     else:
-        order = torch.argsort(row64 * max(1, n_cols) + col64)
+        order = torch.argsort(row64 * max(1, n_cols) + col64, stable=True)
         row_s, col_s, data_s = row64[order], col64[order], data[order]
     indptr = torch.zeros(n_rows + 1, dtype=torch.int64, device=data.device)
     indptr[1:] = torch.cumsum(torch.bincount(row_s, minlength=n_rows), dim=0)

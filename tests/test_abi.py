@@ -110,6 +110,37 @@ def test_spmv_rejects_bad_arguments(csr, plan, code, msg):
     assert msg in _lib.load().fs_last_error().decode()
 
 
+@pytest.mark.parametrize("tile", [
+    dict(),                                    # row groups
+    dict(tile_nnz=2048, n_tiles=1, tile_start=8, tile_mode=_lib.FS_TILE_ROWS),  # row tiles
+    dict(n_long=1, long_rows=8, long_chunk_ptr=8, chunk_long=8, n_chunks=1, chunk_nnz=64,
+         partials=8, partials_bytes=64),       # chunked long rows
+])
+def test_row_sums_only_with_entry_tiles(tile):
+    """A plan that asks for raw row sums outside the entry-tile role is
+    refused before launch, also when y is null (that role would store
+    through y)."""
+    lib = _lib.load()
+    csr = _csr()
+    plan = _plan(row_sums=128, **tile)
+    for fn in (lambda: lib.fs_spmv_csr(ctypes.byref(csr), 64, None, 1.0, 0.0,
+                                       ctypes.byref(plan), None),
+               lambda: lib.fs_spmv_csr_split(ctypes.byref(csr), _lib.FS_STORE_BF16X2, 64, 72,
+                                             None, 80, None, 1.0, 0.0, ctypes.byref(plan),
+                                             None)):
+        assert fn() == _lib.FS_ERR_VALUE
+        assert "row_sums" in lib.fs_last_error().decode()
+
+
+def test_batch_queue_bound():
+    """The batched work queue counts in int32: item counts that would wrap it
+    are refused before launch."""
+    lib = _lib.load()
+    rc = lib.fs_batch_run(8, 1, 16, 2**31 - 10, 4096, 24, 256, 1.0, 0.0, None)
+    assert rc == _lib.FS_ERR_UNSUPPORTED
+    assert "2^31" in lib.fs_last_error().decode()
+
+
 def test_group_lanes_checked_before_launch():
     # a bad group width is reported without touching the device (n_rows > 0,
     # pointers are never dereferenced on the host)

@@ -157,6 +157,36 @@ def test_spmm_alpha_beta_noncontiguous_out():
     assert np.allclose(host64(res), -1.5 * host64(base) + 2.0 * C0, rtol=1e-5, atol=1e-5)
 
 
+def test_spmm_beta_with_strided_out():
+    """A strided out (a transposed view and a column slice) with beta != 0:
+    C = alpha A B + beta out, on the ALG1, opt and alg2 paths."""
+    rng = np.random.default_rng(19)
+    ip, cols, vals = _rand_csr(rng, 300, 260, 6.0, np.float32)
+    B = to_dev(rng.uniform(-1, 1, (260, 12)).astype(np.float32))
+    args = (to_dev(vals), to_dev(cols), to_dev(ip))
+    base = host64(flagsparse_spmm_csr(*args, B, (300, 260)))
+    C0 = rng.uniform(-1, 1, (300, 12)).astype(np.float32)
+    pm = prepare_spmm_csr_opt(*args, (300, 260))
+    p2 = prepare_spmm_csr_opt_alg2(*args, (300, 260))
+    runs = [lambda out: flagsparse_spmm_csr(*args, B, (300, 260), out=out, alpha=0.5, beta=-2.0),
+            lambda out: flagsparse_spmm_csr_opt(B=B, prepared=pm, out=out, alpha=0.5, beta=-2.0),
+            lambda out: flagsparse_spmm_csr_opt_alg2(B=B, prepared=p2, out=out, alpha=0.5,
+                                                     beta=-2.0)]
+    for run in runs:
+        holder = to_dev(np.ascontiguousarray(C0.T))   # (12, 300)
+        out = holder.t()                                # (300, 12), strided
+        assert not out.is_contiguous()
+        assert run(out) is out
+        assert np.allclose(host64(out), 0.5 * base - 2.0 * C0, rtol=1e-5, atol=1e-5)
+        wide = torch.full((300, 20), 9.0, device=DEV)
+        wide[:, 4:16] = to_dev(C0)
+        out = wide[:, 4:16]
+        assert not out.is_contiguous()
+        assert run(out) is out
+        assert np.allclose(host64(out), 0.5 * base - 2.0 * C0, rtol=1e-5, atol=1e-5)
+        assert torch.all(wide[:, :4] == 9.0) and torch.all(wide[:, 16:] == 9.0)
+
+
 def test_spmm_api_validation():
     """The reference's negative API cases (tests/test_spmm.py:728-748)."""
     d = torch.tensor([1.0, 2.0, 3.0], device=DEV)

@@ -323,6 +323,37 @@ def test_spmv_alpha_beta_and_out():
     assert res2 is out2 and not torch.isnan(res2).any()
 
 
+@pytest.mark.parametrize("use_opt", [True, False], ids=["opt", "default"])
+def test_spmv_alpha_without_out_and_strided_out(use_opt):
+    """alpha applies with out=None; a strided out gets alpha A x + beta out
+    (staged through a contiguous copy) and the entries between its elements
+    stay untouched."""
+    n = 2500
+    ip, cols, vals = synthetic.poisson_csr(n, 9.0, 13, lo=0)
+    x = synthetic.dense_uniform(14, n)
+    y0 = synthetic.dense_uniform(15, n)
+    data, indices, indptr, xd = to_dev(vals), to_dev(cols), to_dev(ip), to_dev(x)
+    p = prepare_spmv_csr(data, indices, indptr, (n, n))
+    base = host64(flagsparse_spmv_csr(x=xd, prepared=p, use_opt=use_opt))
+    y = flagsparse_spmv_csr(x=xd, prepared=p, use_opt=use_opt, alpha=2.0)
+    assert np.allclose(host64(y), 2.0 * base, rtol=1e-6, atol=1e-6)
+    y = flagsparse_spmv_csr(data, indices, indptr, xd, (n, n), use_opt=use_opt, alpha=-3.0)
+    assert np.allclose(host64(y), -3.0 * base, rtol=1e-6, atol=1e-6)
+    buf = torch.full((2 * n,), 7.0, device=DEV)
+    buf[::2] = to_dev(y0)
+    out = buf[::2]
+    assert not out.is_contiguous()
+    res = flagsparse_spmv_csr(x=xd, prepared=p, use_opt=use_opt, out=out, alpha=2.0, beta=-0.5)
+    assert res is out
+    assert np.allclose(host64(res), 2.0 * base - 0.5 * y0.astype(np.float64), rtol=1e-6,
+                       atol=1e-6)
+    assert torch.all(buf[1::2] == 7.0)
+    # beta = 0: the strided out is written, not read
+    buf[::2] = float("nan")
+    res = flagsparse_spmv_csr(x=xd, prepared=p, use_opt=use_opt, out=out, alpha=0.5)
+    assert np.allclose(host64(res), 0.5 * base, rtol=1e-6, atol=1e-6)
+
+
 def test_spmv_empty_shapes():
     e = torch.empty(0, device=DEV)
     y = flagsparse_spmv_csr(e, torch.empty(0, dtype=torch.int32, device=DEV),

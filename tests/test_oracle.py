@@ -84,6 +84,83 @@ def test_spmm_bitexact_vs_scipy(orc, nd):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("nd", [1, 5, 32])
+def test_spmm_complex_matches_scipy(orc, nd):
+    """Complex SpMM against SciPy's complex csr_matvecs (same products, same
+    order): only fused-vs-unfused rounding may differ."""
+    rng = np.random.default_rng(17 + nd)
+    ip, cols, vals = rand_csr(rng, 130, 70, 0.1, dup=True, unsorted=True)
+    cv = vals + 1j * rng.uniform(-1, 1, vals.size)
+    B = rng.uniform(-1, 1, (70, nd)) + 1j * rng.uniform(-1, 1, (70, nd))
+    ref = sp.csr_matrix((cv, cols, ip), shape=(130, 70)) @ B
+    got = orc.spmm(ip, cols, cv, B)
+    assert got.shape == ref.shape and got.dtype == np.complex128
+    assert np.max(np.abs(got - ref)) <= 1e-15 * max(1.0, np.max(np.abs(ref)))
+    # a column of B is the complex SpMV with that column
+    for k in (0, nd - 1):
+        yk = orc.spmv(ip, cols, cv, B[:, k])
+        assert np.max(np.abs(got[:, k] - yk)) <= 1e-15 * max(1.0, np.max(np.abs(yk)))
+
+
+def test_spmm_complex_known_answer(orc):
+    """[[1+2j, 3-1j]] @ [[2-1j, 1], [1+1j, 1j]] = [[8+5j, 2+5j]], exactly: a
+    dropped or sign-flipped imaginary cross term changes it."""
+    ip = np.array([0, 2])
+    cols = np.array([0, 1])
+    cv = np.array([1 + 2j, 3 - 1j])
+    B = np.array([[2 - 1j, 1 + 0j], [1 + 1j, 1j]])
+    C = orc.spmm(ip, cols, cv, B)
+    assert np.array_equal(C, np.array([[8 + 5j, 2 + 5j]]))
+    assert np.array_equal(orc.spmv(ip, cols, cv, B[:, 0]), np.array([8 + 5j]))
+
+
+@pytest.mark.parametrize("dup,unsorted", [(False, False), (True, True)])
+def test_condition_sums_bitexact_vs_scipy(orc, dup, unsorted):
+    """sum_j |a_ij x_j| and sum_j |a_ij B_jk| are |A| |x| and |A| |B|: for
+    real fp64 values |a x| = |a| |x| exactly, and SciPy sums in the same
+    order, so the results are bit-identical."""
+    rng = np.random.default_rng(23)
+    ip, cols, vals = rand_csr(rng, 240, 150, 0.06, dup=dup, unsorted=unsorted)
+    x = rng.uniform(-1, 1, 150)
+    B = rng.uniform(-1, 1, (150, 9))
+    # built from |data| directly: SciPy's abs() first merges repeated entries
+    absA = sp.csr_matrix((np.abs(vals), cols, ip), shape=(240, 150))
+    assert np.array_equal(orc.spmv_abs(ip, cols, vals, x), absA @ np.abs(x))
+    assert np.array_equal(orc.spmm_abs(ip, cols, vals, B), absA @ np.abs(B))
+    # not the magnitude of the sum: cancellation shows the difference
+    assert np.all(orc.spmv_abs(ip, cols, vals, x) >= np.abs(orc.spmv(ip, cols, vals, x)))
+
+
+def test_condition_sums_complex_vs_scipy(orc):
+    """Complex: |a x| = |a| |x| up to the rounding of the two moduli, so the
+    sums agree with SciPy's |A| |x| within a few ulps per term."""
+    rng = np.random.default_rng(29)
+    ip, cols, vals = rand_csr(rng, 160, 110, 0.08, dup=True)
+    cv = vals + 1j * rng.uniform(-1, 1, vals.size)
+    x = rng.uniform(-1, 1, 110) + 1j * rng.uniform(-1, 1, 110)
+    B = rng.uniform(-1, 1, (110, 6)) + 1j * rng.uniform(-1, 1, (110, 6))
+    absA = sp.csr_matrix((np.abs(cv), cols, ip), shape=(160, 110))
+    ref_v = absA @ np.abs(x)
+    ref_m = absA @ np.abs(B)
+    got_v = orc.spmv_abs(ip, cols, cv, x)
+    got_m = orc.spmm_abs(ip, cols, cv, B)
+    assert got_v.dtype == np.float64 and got_m.dtype == np.float64
+    assert np.all(np.abs(got_v - ref_v) <= 8 * 2.0 ** -53 * ref_v)
+    assert np.all(np.abs(got_m - ref_m) <= 8 * 2.0 ** -53 * ref_m)
+
+
+def test_condition_sum_known_answer(orc):
+    """A = [[1, -2], [0, 0.5]], x = [3, 4]: y = [-5, 2], sums [11, 2]."""
+    ip, cols, vals = np.array([0, 2, 3]), np.array([0, 1, 1]), np.array([1.0, -2.0, 0.5])
+    x = np.array([3.0, 4.0])
+    assert np.array_equal(orc.spmv(ip, cols, vals, x), [-5.0, 2.0])
+    assert np.array_equal(orc.spmv_abs(ip, cols, vals, x), [11.0, 2.0])
+    B = np.array([[3.0, -1.0], [4.0, 2.0]])
+    assert np.array_equal(orc.spmm_abs(ip, cols, vals, B), [[11.0, 5.0], [2.0, 1.0]])
+    cx = np.array([3j, 4.0])
+    assert np.array_equal(orc.spmv_abs(ip, cols, vals.astype(complex), cx), [11.0, 2.0])
+
+
 @pytest.mark.parametrize("name", GOLDEN_NAMES)
 def test_oracle_against_exact_golden(orc, name):
     c = golden_case(name)
