@@ -28,6 +28,7 @@ FS_ACC_NATIVE, FS_ACC_WIDE = 0, 1
 FS_STORE_BF16X2, FS_STORE_BF16, FS_STORE_F16_ROWSCALE = 1, 2, 3
 FS_STATS_LEN = 16
 FS_TILE_ROWS, FS_TILE_ENTRIES = 0, 1
+FS_NOALLOC_SCRATCH_BYTES = 4 << 20  # fs_spmv_csr_noalloc's per-GPU partials scratch
 
 _COMPLEX32 = getattr(torch, "complex32", None) or getattr(torch, "chalf", None)
 
@@ -147,6 +148,9 @@ _SIGNATURES = {
     "fs_spmv_csr": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_double, ctypes.c_double, ctypes.POINTER(FsPlan),
                                    ctypes.c_void_p]),
+    "fs_spmv_csr_noalloc": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_double, ctypes.c_double, ctypes.POINTER(FsPlan),
+                                           ctypes.c_void_p]),
     "fs_spmm_csr": (ctypes.c_int, [ctypes.POINTER(FsCsr), ctypes.c_void_p, ctypes.c_int64,
                                    ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                    ctypes.c_double, ctypes.c_double, ctypes.POINTER(FsPlan),

@@ -251,6 +251,22 @@ int fs_spmv_csr(const fs_csr *A, const void *x, void *y, double alpha, double be
                 const fs_plan *plan, void *stream);
 
 /*
+ * fs_spmv_csr without a caller-owned partials buffer: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * plan->partials and plan->partials_bytes are ignored; the chunk partials /
+ * tile carries live in a library-owned device array of
+ * FS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends.  A plan that
+ * needs more (fs_tile_carry_bytes / fs_partials_bytes) is rejected with
+ * FS_ERR_UNSUPPORTED before anything is launched.  The call allocates
+ * nothing.  Calls that overlap on one GPU (several streams or host threads)
+ * share the array and race; order them on one stream.
+ */
+#define FS_NOALLOC_SCRATCH_BYTES (4u << 20)
+int fs_spmv_csr_noalloc(const fs_csr *A, const void *x, void *y, double alpha, double beta,
+                        const fs_plan *plan, void *stream);
+
+/*
  * C = alpha * A B + beta * C with B (n_cols x n_dense, leading dimension ldb)
  * and C (n_rows x n_dense, leading dimension ldc), both row-major.  C is not
  * read when beta == 0.  col_tile is the number of dense columns one lane

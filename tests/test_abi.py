@@ -141,6 +141,27 @@ def test_batch_queue_bound():
     assert "2^31" in lib.fs_last_error().decode()
 
 
+def test_noalloc_scratch_bound():
+    """fs_spmv_csr_noalloc: the scratch size the header states is the one the
+    binding uses, and entry tiles whose carries exceed it are refused before
+    anything is launched (the pointers are never dereferenced)."""
+    text = open(HEADER).read()
+    m = re.search(r"#define FS_NOALLOC_SCRATCH_BYTES \((\d+)u << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == _lib.FS_NOALLOC_SCRATCH_BYTES
+    lib = _lib.load()
+    n_tiles = _lib.FS_NOALLOC_SCRATCH_BYTES // 16 + 1  # fp32 carries: 16 bytes per tile
+    csr = _csr(ptr_type=_lib.FS_I64, n_rows=n_tiles, n_cols=8, nnz=n_tiles * 1024)
+    plan = _plan(short_max=0, chunk_nnz=64, tile_nnz=1024, n_tiles=n_tiles, tile_start=96,
+                 tile_mode=_lib.FS_TILE_ENTRIES, tile_entry=112)
+    rc = lib.fs_spmv_csr_noalloc(ctypes.byref(csr), 64, 80, 1.0, 0.0, ctypes.byref(plan), None)
+    assert rc == _lib.FS_ERR_UNSUPPORTED
+    assert "scratch" in lib.fs_last_error().decode()
+    # the same plan one tile smaller fits (checked by the size query alone)
+    assert _lib.tile_carry_bytes(_lib.FS_F32, n_tiles - 1) == _lib.FS_NOALLOC_SCRATCH_BYTES
+    assert lib.fs_spmv_csr_noalloc(None, 64, 80, 1.0, 0.0, ctypes.byref(plan), None) == \
+        _lib.FS_ERR_VALUE
+
+
 def test_group_lanes_checked_before_launch():
     # a bad group width is reported without touching the device (n_rows > 0,
     # pointers are never dereferenced on the host)
