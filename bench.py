@@ -19,6 +19,20 @@ dominant kernel is the compact-row SpMV; its own algorithmic bytes are
 for NC compact rows, and its average duration is timed here with HIP events
 on the launch stream, in a loop of its own after the timed steps.
 
+The same JSON line nests, under "workloads", the rest of the metric, each
+timed with HIP events in this run and carrying its error against an fp64
+reference (SciPy's CSR product in float64):
+  * spmv_powerlaw_4m_f64: the same matrix in fp64;
+  * spmm_banded_1m: C = A B on the 1M-row banded matrix (offsets -3..3 plus 8
+    random columns per row), N = 32 / 64 / 128 fp32 and N = 64 fp64, with a
+    compulsory-bytes roofline (A once, B once, C once) and rocSPARSE beside it;
+  * batched_512: 512 uneven matrices, one SpMV + one 8-column SpMM each, as one
+    graph-replayed persistent launch against the 1,024 separate launches;
+  * selection: on the 4M-row SpMV and the N = 64 SpMM, the learned choice
+    against exhaustive autotuning (every variant timed) and the plain
+    default-tile kernel.
+--quick skips them.
+
     python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--path opt|default] [--dtype f32|f64]
 """
 
@@ -346,6 +360,197 @@ def bench_batched(args):
     return 0
 
 
+# ------------------------------------------------------------------ sub-results
+#
+# Every sub-result is timed with HIP events on the launch stream in this run
+# and reports its error against an fp64 reference: SciPy's CSR product on the
+# matrix promoted to float64 (the reference's own correctness oracle is a CSR
+# product in the promoted dtype).  Error measure: max over rows of
+# |got - ref| / sum_j |a_ij x_j| (SpMM: per element, sum_j |a_ij b_jk|).
+
+
+def _scipy_csr(ip, cols, vals, shape):
+    import numpy as np
+    import scipy.sparse as sp
+    return sp.csr_matrix((np.asarray(vals, np.float64), np.asarray(cols, np.int64),
+                          np.asarray(ip, np.int64)), shape=shape)
+
+
+def _rel_err(got, ref, cond):
+    import numpy as np
+    d = np.abs(np.asarray(got, np.float64) - ref)
+    return float((d / np.where(cond > 0, cond, 1.0)).max()) if d.size else 0.0
+
+
+def sub_spmv_f64(steps):
+    """The 4M-row power-law SpMV in fp64 (library schedule, native path)."""
+    import numpy as np
+    import torch
+    from flagsparse_public_amd import flagsparse_spmv_csr, prepare_spmv_csr, synthetic
+    dev = torch.device("cuda", 0)
+    ip, cols, vals = synthetic.spmv_powerlaw_4m(dtype=np.float64)
+    n, nnz = ip.size - 1, int(ip[-1])
+    x = synthetic.dense_uniform(202, n, np.float64)
+    d = [torch.from_numpy(a).to(dev) for a in (vals, cols, ip.astype(np.int32))]
+    dx = torch.from_numpy(x).to(dev)
+    p = prepare_spmv_csr(*d, (n, n))
+    stream = torch.cuda.current_stream(dev)
+    us = _event_time(lambda: flagsparse_spmv_csr(x=dx, prepared=p, use_opt=True), steps, stream)
+    y = flagsparse_spmv_csr(x=dx, prepared=p, use_opt=True).cpu().numpy()
+    A = _scipy_csr(ip, cols, vals, (n, n))
+    ref = A @ x
+    cond = abs(A) @ np.abs(x)
+    nbytes = spmv_bytes(n, nnz, s_val=8, s_x=8, s_y=8)
+    _log(f"spmv fp64: {us:.1f} us")
+    return {"us": round(us, 2), "GB/s": round(nbytes / us / 1e3, 1),
+            "frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4), "bytes": nbytes,
+            "schedule": repr(p.schedule), "chosen_by": p.schedule_record["chosen_by"],
+            "max_rel_err_vs_fp64": _rel_err(y, ref, cond)}
+
+
+def sub_spmm_c3(steps, no_vendor=False):
+    """C = A B on the 1M-row banded matrix for N = 32 / 64 / 128 (fp32) and
+    N = 64 (fp64), each with the schedule the library chooses."""
+    import numpy as np
+    import torch
+    from flagsparse_public_amd import (flagsparse_spmm_csr_opt, prepare_spmm_csr_opt, synthetic,
+                                       vendor)
+    from flagsparse_public_amd.sparse_operations.spmm_csr import spmm_launch_schedule
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    out = {}
+    for N, dt in ((32, "f32"), (64, "f32"), (128, "f32"), (64, "f64")):
+        npd = np.float32 if dt == "f32" else np.float64
+        ip, cols, vals = synthetic.spmm_banded_1m(dtype=npd)
+        n, nnz = ip.size - 1, int(ip[-1])
+        s_v = np.dtype(npd).itemsize
+        d = [torch.from_numpy(a).to(dev) for a in (vals, cols, ip.astype(np.int32))]
+        Bh = synthetic.dense_uniform(303, (n, N), npd)
+        B = torch.from_numpy(Bh).to(dev)
+        p = prepare_spmm_csr_opt(*d, (n, n))
+        launch = spmm_launch_schedule(p, N)
+        us = _event_time(lambda: flagsparse_spmm_csr_opt(B=B, prepared=p), steps, stream)
+        C = flagsparse_spmm_csr_opt(B=B, prepared=p).cpu().numpy()
+        A = _scipy_csr(ip, cols, vals, (n, n))
+        ref = A @ Bh.astype(np.float64)
+        cond = abs(A) @ np.abs(Bh.astype(np.float64))
+        gather = (n + 1) * 4 + nnz * (4 + s_v + N * s_v) + n * N * s_v
+        compulsory = (n + 1) * 4 + nnz * (4 + s_v) + n * N * s_v + n * N * s_v
+        rec = {"us": round(us, 2), "GB/s_gather_counted": round(gather / us / 1e3, 1),
+               "GB/s_compulsory": round(compulsory / us / 1e3, 1),
+               "frac_compulsory": round(compulsory / us / 1e3 / HBM_PEAK_GBS, 4),
+               "gflops": round(2.0 * nnz * N / us / 1e3, 1),
+               "schedule": repr(launch[0]), "col_tile": launch[1],
+               "chosen_by": p.schedule_record["chosen_by"],
+               "max_rel_err_vs_fp64": _rel_err(C, ref, cond)}
+        if not no_vendor and vendor.available():
+            Cv = torch.empty((n, N), dtype=B.dtype, device=dev)
+            v = vendor.VendorSpmm(*d, (n, n), B, Cv)
+            vus = _event_time(v.run, steps, stream)
+            rec["rocsparse_us"] = round(vus, 2)
+        out[f"n{N}_{dt}"] = rec
+        _log(f"spmm N={N} {dt}: {us:.1f} us")
+        del p, B, d, A, ref, cond, C
+        torch.cuda.empty_cache()
+    return out
+
+
+def sub_batched(steps):
+    """512 uneven matrices (one SpMV, one 8-column SpMM each): graph replay of
+    the persistent launch, the same eagerly, and the 1,024 separate launches."""
+    import numpy as np
+    import torch
+    from flagsparse_public_amd.autotune import time_launches
+    from flagsparse_public_amd.batched import BatchedCsr
+    from flagsparse_public_amd.batched_workload import uneven_batch
+    from flagsparse_public_amd import prepare_spmm_csr_opt, prepare_spmv_csr
+    from flagsparse_public_amd.sparse_operations.spmm_csr import _hip_spmm_csr_impl_opt_prepared
+    from flagsparse_public_amd.sparse_operations.spmv_csr import _hip_spmv_csr_impl_opt_prepared
+    dev = torch.device("cuda", 0)
+    mats, meta = uneven_batch(512, 4, dev)
+    bat = BatchedCsr(mats)
+    graph = bat.capture()
+    reps = max(steps // 20, 5)
+    us_graph = time_launches(graph.replay, reps=reps, warmup=2)
+    us_eager = time_launches(bat.run, reps=reps, warmup=2)
+    pv = [prepare_spmv_csr(m["data"], m["indices"], m["indptr"], m["shape"]) for m in mats]
+    pm = [prepare_spmm_csr_opt(m["data"], m["indices"], m["indptr"], m["shape"]) for m in mats]
+    ys = [torch.empty(m["shape"][0], device=dev) for m in mats]
+    Cs = [torch.empty((m["shape"][0], 8), device=dev) for m in mats]
+
+    def separate():
+        for m, a, b, y, C in zip(mats, pv, pm, ys, Cs):
+            _hip_spmv_csr_impl_opt_prepared(a, m["x"], out=y)
+            _hip_spmm_csr_impl_opt_prepared(b, m["B"], out=C)
+    us_sep = time_launches(separate, reps=3, warmup=1)
+    graph.replay()
+    torch.cuda.synchronize()
+    err = 0.0
+    for i in (0, 97, 255, 511):
+        m = mats[i]
+        ip = m["indptr"].cpu().numpy()
+        A = _scipy_csr(ip, m["indices"].cpu().numpy(), m["data"].cpu().numpy(), m["shape"])
+        x = m["x"].double().cpu().numpy()
+        Bh = m["B"].double().cpu().numpy()
+        err = max(err, _rel_err(bat.y[i].cpu().numpy(), A @ x, abs(A) @ np.abs(x)),
+                  _rel_err(bat.C[i].cpu().numpy(), A @ Bh, abs(A) @ np.abs(Bh)))
+    _log(f"batched: graph {us_graph:.0f} us, separate {us_sep:.0f} us")
+    return {"matrices": meta["count"], "rows": meta["rows"], "nnz": meta["nnz"],
+            "graph_replay_ms": round(us_graph / 1e3, 4), "eager_ms": round(us_eager / 1e3, 4),
+            "separate_launches_ms": round(us_sep / 1e3, 4),
+            "speedup_vs_separate": round(us_sep / us_graph, 3),
+            "GB/s": round(bat.bytes / us_graph / 1e3, 1), "bytes": bat.bytes,
+            "max_rel_err_vs_fp64": err}
+
+
+def sub_selection(d_spmv, spmv_prepared, steps):
+    """Learned choice vs exhaustive autotuning vs the plain default-tile
+    kernel, on the 4M-row SpMV (16 variants) and the N = 64 banded SpMM
+    (8 variants).  Every variant is timed the way the autotuner times it."""
+    import numpy as np
+    import torch
+    from flagsparse_public_amd import prepare_spmm_csr_opt, synthetic
+    from flagsparse_public_amd.autotune import (SPMM_DEFAULT_INDEX, SPMM_VARIANTS,
+                                                SPMV_DEFAULT_INDEX, SPMV_VARIANTS,
+                                                spmv_heuristic_index, tune_spmm, tune_spmv)
+    dev = torch.device("cuda", 0)
+    out = {}
+    vals, cols, ip, x = d_spmv
+    t = tune_spmv(spmv_prepared, x, reps=5)
+    how = spmv_prepared.schedule_record["chosen_by"]
+    li = int(how.split(":")[1]) if how.startswith("learned:") else None
+    n = spmv_prepared.n_rows
+    out["spmv_powerlaw_4m"] = _selection_record(t, li, SPMV_DEFAULT_INDEX,
+                                                spmv_heuristic_index(n, n, int(vals.numel())),
+                                                how)
+    ipb, colsb, valsb = synthetic.spmm_banded_1m()
+    nb = ipb.size - 1
+    d = [torch.from_numpy(a).to(dev) for a in (valsb, colsb, ipb.astype(np.int32))]
+    B = torch.from_numpy(synthetic.dense_uniform(303, (nb, 64))).to(dev)
+    pm = prepare_spmm_csr_opt(*d, (nb, nb))
+    tm = tune_spmm(pm, B, reps=5)
+    howm = pm.schedule_record["chosen_by"]
+    lm = int(howm.split(":")[1]) if howm.startswith("learned:") else None
+    out["spmm_banded_1m_n64"] = _selection_record(tm, lm, SPMM_DEFAULT_INDEX, None, howm)
+    _log(f"selection: spmv {out['spmv_powerlaw_4m']}, spmm {out['spmm_banded_1m_n64']}")
+    return out
+
+
+def _selection_record(times, learned, default, rule, how):
+    best = int(times.argmin())
+    rec = {"variants_timed": int(times.size), "exhaustive_best": best,
+           "exhaustive_us": round(float(times[best]), 2),
+           "default_tile_us": round(float(times[default]), 2), "chosen_by": how}
+    if learned is not None:
+        rec["learned"] = learned
+        rec["learned_us"] = round(float(times[learned]), 2)
+        rec["learned_over_exhaustive"] = round(float(times[learned] / times[best]), 4)
+        rec["speedup_learned_vs_default_tile"] = round(float(times[default] / times[learned]), 3)
+    if rule is not None:
+        rec["fixed_rule_us"] = round(float(times[rule]), 2)
+    return rec
+
+
 def calibrate_copy(mib, reps):
     """Plain device-to-device copies of a known size: the reference point that
     turns rocprofv3 FETCH_SIZE / WRITE_SIZE readings into bytes."""
@@ -373,6 +578,8 @@ def main(argv=None):
     ap.add_argument("--no-modes", action="store_true",
                     help="skip the split-precision and default-path lines")
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE line")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the headline SpMV line (no nested workloads)")
     ap.add_argument("--workload", default="spmv_powerlaw_4m",
                     choices=["spmv_powerlaw_4m", "spmm_banded_1m", "batched_512",
                              "sddmm_poisson_1m"],
@@ -541,6 +748,20 @@ def main(argv=None):
         for m in modes.values():
             m.pop("_y", None)
         result["config"]["modes"] = modes
+    if not args.quick and args.dtype == "f32" and args.n == 4_000_000:
+        sub_steps = max(min(args.steps, 100), 10)
+        work = {}
+        t_sub = time.time()
+        work["selection"] = sub_selection((d_vals, d_cols, d_ip, d_x), prepared, sub_steps)
+        prepared._plans.clear()  # the variants' layouts
+        torch.cuda.empty_cache()
+        work["spmv_powerlaw_4m_f64"] = sub_spmv_f64(sub_steps)
+        torch.cuda.empty_cache()
+        work["spmm_banded_1m"] = sub_spmm_c3(sub_steps, args.no_vendor)
+        torch.cuda.empty_cache()
+        work["batched_512"] = sub_batched(sub_steps)
+        work["seconds"] = round(time.time() - t_sub, 1)
+        result["workloads"] = work
     print(json.dumps(result), flush=True)
     return 0
 

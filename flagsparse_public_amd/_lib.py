@@ -84,6 +84,7 @@ class FsPlan(ctypes.Structure):
         ("group_tiles", ctypes.c_void_p),
         ("row_sums", ctypes.c_void_p),
         ("tile_entry", ctypes.c_void_p),
+        ("x_perm", ctypes.c_void_p),
     ]
 
 
@@ -116,6 +117,24 @@ class FsSlices(ctypes.Structure):
     ]
 
 
+class FsXwin(ctypes.Structure):
+    _fields_ = [
+        ("value_type", ctypes.c_int32),
+        ("window", ctypes.c_int32),
+        ("n_cols", ctypes.c_int64),
+        ("nnz", ctypes.c_int64),
+        ("n_runs", ctypes.c_int64),
+        ("n_parts", ctypes.c_int64),
+        ("wcol", ctypes.c_void_p),
+        ("batch_run", ctypes.c_void_p),
+        ("batch_mask", ctypes.c_void_p),
+        ("run_off", ctypes.c_void_p),
+        ("part_window", ctypes.c_void_p),
+        ("part_entry", ctypes.c_void_p),
+        ("window_entry", ctypes.c_void_p),
+    ]
+
+
 class FsMtxInfo(ctypes.Structure):
     _fields_ = [
         ("n_rows", ctypes.c_int64),
@@ -141,6 +160,9 @@ _lib_lock = threading.Lock()
 
 # Every symbol the header declares, with its ctypes signature.
 _SIGNATURES = {
+    "fs_xwin_max_window": (ctypes.c_int64, [ctypes.c_int32]),
+    "fs_xwin_gather": (ctypes.c_int, [ctypes.POINTER(FsXwin), ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
     "fs_partials_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, ctypes.c_int64]),
     "fs_tile_carry_bytes": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int64]),
     "fs_sliced_combine": (ctypes.c_int, [ctypes.POINTER(FsSlices), ctypes.c_void_p, ctypes.c_void_p,
@@ -401,6 +423,17 @@ def partials_bytes(value_type: int, accumulate: int, n_chunks: int, n_dense: int
 def sliced_combine(sl: FsSlices, z, y, alpha: float, beta: float, device) -> None:
     check(load().fs_sliced_combine(ctypes.byref(sl), ptr(z), ptr(y), float(alpha), float(beta),
                                    stream_handle(device)), "fs_sliced_combine")
+
+
+def xwin_max_window(value_type: int, device) -> int:
+    with on_device(device):
+        return int(load().fs_xwin_max_window(int(value_type)))
+
+
+def xwin_gather(layout: FsXwin, x, xg, device) -> None:
+    with on_device(device):
+        check(load().fs_xwin_gather(ctypes.byref(layout), ptr(x), ptr(xg), stream_handle(device)),
+              "fs_xwin_gather")
 
 
 def sddmm(csr: FsCsr, row_ids, x, y, k_dim: int, out, out_type: int, accumulate: int,

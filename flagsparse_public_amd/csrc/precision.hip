@@ -89,9 +89,28 @@ __global__ __launch_bounds__(256) void encode_f16_rowscale(const P* __restrict__
 
 // ---------------------------------------------------------------- launches
 
+// Entry tiles over x values gathered per entry (fs_plan.x_perm).
+template <typename VS, typename P>
+int launch_split_xg(const fs_csr* M, const VS& vs, const void* xg, void* y, double alpha,
+                    double beta, const fs_plan* pl, hipStream_t st) {
+  SpmvParams<VS, float, uint16_t, P> p;
+  p.ptr = static_cast<const P*>(M->row_ptr);
+  p.idx = pl->x_perm;
+  p.vals = vs;
+  p.x = static_cast<const float*>(xg);
+  p.y = static_cast<float*>(y);
+  p.alpha = alpha;
+  p.beta = beta;
+  const int64_t grid = spmv_fill_schedule(p, M, pl, 1);
+  if (grid > 0x7fffffffLL)
+    return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr_split: grid of %lld blocks too large", (long long)grid);
+  return spmv_launch_entry<Plain<float>, false, true>(p, grid, st, "fs_spmv_csr_split");
+}
+
 template <typename VS, typename P, int S>
 int launch_split_s(const fs_csr* M, const VS& vs, const void* x, void* y, double alpha,
                    double beta, const fs_plan* pl, hipStream_t st) {
+  if (pl->x_perm) return launch_split_xg<VS, P>(M, vs, x, y, alpha, beta, pl, st);
   SpmvParams<VS, float, int32_t, P> p;
   p.ptr = static_cast<const P*>(M->row_ptr);
   p.idx = static_cast<const int32_t*>(M->col_idx);

@@ -63,9 +63,32 @@ int launch_spmv_s(const fs_csr* M, const void* x, void* y, double alpha, double 
   return rc ? rc : spmv_launch_fixup<A>(p, pl, st, "fs_spmv_csr");
 }
 
+// Entry tiles over x values gathered per entry (fs_plan.x_perm, fp32 / fp64):
+// the permutation takes the place of the column indices.
+template <typename V, typename A, typename P>
+int launch_spmv_entries_xg(const fs_csr* M, const void* xg, void* y, double alpha, double beta,
+                           const fs_plan* pl, hipStream_t st) {
+  using VS = DirectVals<V>;
+  SpmvParams<VS, V, uint16_t, P> p;
+  p.ptr = static_cast<const P*>(M->row_ptr);
+  p.idx = pl->x_perm;
+  p.vals.v = static_cast<const V*>(M->values);
+  p.x = static_cast<const V*>(xg);
+  p.y = static_cast<V*>(y);
+  p.alpha = alpha;
+  p.beta = beta;
+  const int64_t grid = spmv_fill_schedule(p, M, pl, 1);
+  if (grid > 0x7fffffffLL)
+    return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr: grid of %lld blocks too large", (long long)grid);
+  return spmv_launch_entry<A, true, true>(p, grid, st, "fs_spmv_csr");
+}
+
 template <typename V, typename A, typename I, typename P>
 int launch_spmv_entries(const fs_csr* M, const void* x, void* y, double alpha, double beta,
                         const fs_plan* pl, hipStream_t st) {
+  if constexpr (std::is_same<V, float>::value || std::is_same<V, double>::value) {
+    if (pl->x_perm) return launch_spmv_entries_xg<V, A, P>(M, x, y, alpha, beta, pl, st);
+  }
   using VS = DirectVals<V>;
   SpmvParams<VS, V, I, P> p;
   p.ptr = static_cast<const P*>(M->row_ptr);

@@ -474,6 +474,20 @@ template <typename C> __host__ __device__ inline size_t entry_lds_bytes(int64_t 
          sizeof(S) * kWavesPerBlock;
 }
 
+// Gathered-x form (fs_plan.x_perm): the tile's gathered x values (type V)
+// are staged where the products go (they are read into registers, then a
+// barrier, before any product is written) when V is no wider than the
+// product type C; otherwise they follow the entry role's LDS, 16-byte aligned.
+template <typename C, typename V> constexpr bool kXgAliased = sizeof(V) <= sizeof(C);
+template <typename C, typename V> __host__ __device__ inline size_t entry_xg_offset(int64_t tile_nnz) {
+  if constexpr (kXgAliased<C, V>) return 0;
+  return (entry_lds_bytes<C>(tile_nnz) + 15) & ~(size_t)15;
+}
+template <typename C, typename V> __host__ __device__ inline size_t entry_xg_lds_bytes(int64_t tile_nnz) {
+  if constexpr (kXgAliased<C, V>) return entry_lds_bytes<C>(tile_nnz);
+  return entry_xg_offset<C, V>(tile_nnz) + sizeof(V) * (size_t)tile_nnz;
+}
+
 // A finished row of the entry role: y = alpha * sum + beta * y, or, with
 // row_sums set, the raw sum in the sum type (times the row scale of scaled
 // storage) for a later pass to combine.
@@ -604,6 +618,51 @@ __device__ __forceinline__ void entry_gather(const SpmvParams<VS, V, I, P>& p,
   }
 }
 
+// Gathered-x form: the tile's x values, already gathered per entry, lie at
+// x[e0 .. e0 + len) in the order of x_perm (f.col holds each entry's x_perm
+// value).  They are staged in LDS with coalesced loads -- all issued before
+// the first LDS store -- and then read through the permutation (clamped into
+// the tile, so a damaged permutation cannot leave it).
+template <int E, typename VS, typename V, typename I, typename P, typename C>
+__device__ __forceinline__ void entry_stage_xg(const SpmvParams<VS, V, I, P>& p,
+                                               const EntryTile<I, C, E>& f, V* xt) {
+  const int tid = threadIdx.x;
+  if (f.len == 0) return;  // uniform: an empty tile (e0 may equal nnz) reads nothing
+  if (f.vec) {
+#pragma unroll
+    for (int g = 0; g < E / 4; ++g) {
+      const int o = (g * kBlock + tid) * 4;
+      V v[4];
+      load4(p.x, f.e0 + o, v);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) xt[o + k] = v[k];
+    }
+  } else {
+    V v[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int o = u * kBlock + tid;
+      v[u] = p.x[f.e0 + (o < f.len ? o : 0)];
+    }
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const int o = u * kBlock + tid;
+      if (o < f.len) xt[o] = v[u];
+    }
+  }
+}
+
+template <int E, typename I, typename C, typename V>
+__device__ __forceinline__ void entry_gather_xg(const EntryTile<I, C, E>& f, const V* xt,
+                                                C (&xv)[E]) {
+  constexpr int T = E * kBlock;
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int q = (int)f.col[u];
+    xv[u] = load_as<C>(xt, q < T ? q : T - 1);
+  }
+}
+
 // Products to LDS in entry order, and the tile's row boundaries (entry
 // offsets clamped to [0, len]) of its first batch of owned rows.
 template <int E, typename I, typename C>
@@ -713,7 +772,7 @@ template <typename C, typename S, int T> struct EntryLds {
 
 // One tile per block (block b takes tile b, or with block groups the b / G-th
 // tile of group b % G).
-template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>
+template <typename A, int E, bool VEC, bool XG, typename VS, typename V, typename I, typename P>
 __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I, P> p) {
   using C = typename A::compute_t;
   using S = typename TileSum<C>::T;
@@ -729,7 +788,15 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I,
   EntryTile<I, C, E> f;
   entry_fetch<E, VEC>(p, t, f);
   C xv[E];
-  entry_gather(p, f, xv);
+  if constexpr (XG) {
+    V* xt = reinterpret_cast<V*>(smem + entry_xg_offset<C, V>(E * kBlock));
+    entry_stage_xg(p, f, xt);
+    __syncthreads();
+    entry_gather_xg(f, xt, xv);
+    if constexpr (kXgAliased<C, V>) __syncthreads();  // every read of xt before the products
+  } else {
+    entry_gather(p, f, xv);
+  }
   entry_stage(f, xv, lds.prod, lds.bnd);
   __syncthreads();
   entry_rows<A>(p, f, lds.prod, lds.bnd, lds.red, carries);
@@ -755,10 +822,11 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS,
   entry_write(p, r1 - 1, acc);
 }
 
-template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>
+template <typename A, int E, bool VEC, bool XG, typename VS, typename V, typename I, typename P>
 void spmv_launch_entry_e(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st) {
-  spmv_entry_kernel<A, E, VEC><<<dim3((unsigned)grid), dim3(kBlock),
-                                 entry_lds_bytes<typename A::compute_t>(E * kBlock), st>>>(p);
+  using C = typename A::compute_t;
+  const size_t lds = XG ? entry_xg_lds_bytes<C, V>(E * kBlock) : entry_lds_bytes<C>(E * kBlock);
+  spmv_entry_kernel<A, E, VEC, XG><<<dim3((unsigned)grid), dim3(kBlock), lds, st>>>(p);
 }
 
 // Whether the col / value arrays allow the 16-byte loads of the entry role.
@@ -774,24 +842,30 @@ bool entry_vec_ok(const SpmvParams<VS, V, I, P>& p) {
 
 // Launches the entry-tile kernel and its fix-up.  HOT builds tiles of 1024,
 // 2048 and 4096 entries with and without vector loads; others 2048, scalar.
-template <typename A, bool HOT, typename VS, typename V, typename I, typename P>
+// XG: the gathered-x form (x holds per-entry x values, p.idx is x_perm).
+template <typename A, bool HOT, bool XG = false, typename VS, typename V, typename I, typename P>
 int spmv_launch_entry(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st,
                       const char* who) {
   using C = typename A::compute_t;
-  if (entry_lds_bytes<C>(p.tile_nnz) > 65536)
+  const size_t lds = XG ? entry_xg_lds_bytes<C, V>(p.tile_nnz) : entry_lds_bytes<C>(p.tile_nnz);
+  if (lds > 65536)
     return fail(FS_ERR_VALUE, "%s: entry tiles of %lld entries need more than 64 KB of LDS", who,
                 (long long)p.tile_nnz);
-  const bool vec = entry_vec_ok(p);
+  bool vec = entry_vec_ok(p);
+  if constexpr (XG) vec = vec && ((uintptr_t)p.x % 16) == 0;
   bool done = false;
   if constexpr (HOT) {
     switch (p.tile_nnz) {
-      case 1024: vec ? spmv_launch_entry_e<A, 4, true>(p, grid, st) : spmv_launch_entry_e<A, 4, false>(p, grid, st); done = true; break;
-      case 2048: vec ? spmv_launch_entry_e<A, 8, true>(p, grid, st) : spmv_launch_entry_e<A, 8, false>(p, grid, st); done = true; break;
-      case 4096: vec ? spmv_launch_entry_e<A, 16, true>(p, grid, st) : spmv_launch_entry_e<A, 16, false>(p, grid, st); done = true; break;
+      case 1024: vec ? spmv_launch_entry_e<A, 4, true, XG>(p, grid, st) : spmv_launch_entry_e<A, 4, false, XG>(p, grid, st); done = true; break;
+      case 2048: vec ? spmv_launch_entry_e<A, 8, true, XG>(p, grid, st) : spmv_launch_entry_e<A, 8, false, XG>(p, grid, st); done = true; break;
+      case 4096:
+        vec ? spmv_launch_entry_e<A, 16, true, XG>(p, grid, st) : spmv_launch_entry_e<A, 16, false, XG>(p, grid, st);
+        done = true;
+        break;
       default: break;
     }
   } else if (p.tile_nnz == 2048) {
-    spmv_launch_entry_e<A, 8, false>(p, grid, st);
+    spmv_launch_entry_e<A, 8, false, XG>(p, grid, st);
     done = true;
   }
   if (!done)

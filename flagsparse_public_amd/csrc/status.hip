@@ -69,7 +69,9 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
   if (P->tile_nnz < 0 || P->n_tiles < 0) return fail(FS_ERR_VALUE, "%s: negative tile fields", who);
   if (P->tile_mode != FS_TILE_ROWS && P->tile_mode != FS_TILE_ENTRIES)
     return fail(FS_ERR_VALUE, "%s: tile_mode must be FS_TILE_ROWS or FS_TILE_ENTRIES", who);
-  // only the entry-tile role stores raw row sums; every other role writes y
+  // only the entry-tile role stores raw row sums or reads gathered x
+  if (P->x_perm && !(P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES))
+    return fail(FS_ERR_VALUE, "%s: x_perm is offered for entry tiles only", who);
   if (P->row_sums && !(P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES))
     return fail(FS_ERR_VALUE, "%s: row_sums are offered for entry tiles only", who);
   if (P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES) {
@@ -86,6 +88,9 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
     if (A->value_type != FS_F32 && A->value_type != FS_F64 && A->value_type != FS_C64 &&
         A->value_type != FS_C128 && P->row_sums)
       return fail(FS_ERR_VALUE, "%s: row sums are offered for fp32 / fp64 / complex values", who);
+    if (P->x_perm && ((A->value_type != FS_F32 && A->value_type != FS_F64) || P->row_sums))
+      return fail(FS_ERR_VALUE, "%s: gathered x (x_perm) is offered for fp32 / fp64 values "
+                  "without row_sums", who);
     const int64_t need = fs_tile_carry_bytes(A->value_type, P->n_tiles);
     if (!P->partials || P->partials_bytes < need)
       return fail(FS_ERR_VALUE, "%s: carry buffer too small (%lld < %lld bytes)", who,

@@ -34,6 +34,8 @@ class Schedule:
     x_slice_cols: int = 0     # SpMV native path: > 0 cuts the columns into slices of
                               # at most this many (one launch per slice, x slice in L2)
     tile_mode: int = 0        # SpMV with tile_nnz > 0: 0 row tiles, 1 entry tiles
+    x_window: int = 0         # SpMV fp32 / fp64 with entry tiles: != 0 gathers x from
+                              # windows staged in LDS first (columns per window; -1: 128 KB)
 
     def key(self) -> "Schedule":
         """Cache key (a frozen dataclass hashes and compares by its fields)."""
@@ -134,6 +136,7 @@ class RowSplitPlan:
         self._structs = {}
         self.n_groups, self.group_blocks, self.group_tiles = 0, 0, None
         self.row_sums = None
+        self.x_perm = None
         if schedule.tile_nnz > 0 and schedule.tile_mode == _lib.FS_TILE_ENTRIES:
             self._init_entry_tiles(indptr, n_rows, schedule)
             return
@@ -224,6 +227,12 @@ class RowSplitPlan:
         self.row_sums = sums
         self._structs.clear()
 
+    def set_x_perm(self, perm: torch.Tensor):
+        """Entry tiles: x is gathered per entry (x windows); entry j of tile t
+        reads x[tile_entry[t] + perm[j]]."""
+        self.x_perm = perm
+        self._structs.clear()
+
     def struct(self, n_dense: int = 1) -> _lib.FsPlan:
         """fs_plan for a call with n_dense dense columns (1 for SpMV)."""
         key = max(int(n_dense), 1)
@@ -248,7 +257,8 @@ class RowSplitPlan:
                              group_blocks=self.group_blocks,
                              group_tiles=_lib.ptr(self.group_tiles),
                              row_sums=_lib.ptr(self.row_sums),
-                             tile_entry=_lib.ptr(self.tile_entry))
+                             tile_entry=_lib.ptr(self.tile_entry),
+                             x_perm=_lib.ptr(self.x_perm))
             self._structs[key] = st
             return st
         if self.n_long:

@@ -26,6 +26,7 @@ from .. import _lib
 from ..rowsplit import RowSplitPlan, Schedule, default_spmv_schedule
 from ..schedule_choice import choose_schedule, exhaustive_spmv_schedule
 from ..slicing import SLICED_DTYPES, SlicedCsr
+from ..xwin import XWIN_DTYPES, XWinCsr
 from ._common import (
     SUPPORTED_INDEX_DTYPES,
     _INDEX_LIMIT_INT32,
@@ -382,6 +383,19 @@ def _launch_spmv(prepared, x, accumulate, out=None, alpha=1.0, beta=0.0, schedul
     if prepared.n_rows == 0:
         return y
     sch = schedule or prepared.schedule
+    if (sch is not None and sch.x_window != 0 and prepared.data.dtype in XWIN_DTYPES
+            and prepared.data.numel() > 0):
+        # two streaming passes: x gathered from LDS windows, then entry tiles
+        key = ("xwin", sch.key())
+        xw = prepared._plans.get(key)
+        if xw is None:
+            xw = XWinCsr(prepared.data, prepared.kernel_indices, prepared.kernel_indptr,
+                         prepared.n_rows, prepared.n_cols,
+                         tile_nnz=sch.tile_nnz if sch.tile_nnz > 0 else 2048,
+                         window=sch.x_window if sch.x_window > 0 else None)
+            prepared._plans[key] = xw
+        xw.run(x, y, alpha, beta, accumulate, prepared.data.device)
+        return y
     if (sch is not None and sch.x_slice_cols > 0 and prepared.n_cols > sch.x_slice_cols
             and prepared.data.dtype in SLICED_DTYPES):
         # native partials in the value dtype; the default fp32 path (fp64

@@ -1,0 +1,228 @@
+"""x-window SpMV: random x gathers served from LDS instead of L2.
+
+At ~12 entries per row over a 16 MB x, a one-pass SpMV issues one L2 request
+per 4-byte x gather, and the L2 request rate, not HBM bandwidth, bounds it.
+This schedule splits the SpMV into two streaming passes:
+
+1. ``fs_xwin_gather``: the columns are cut into windows of W columns (one
+   window of x fits one workgroup's LDS: W = 32768 fp32 / 16384 fp64 on
+   MI355X's 160 KB).  A second copy of the column indices, stored
+   window-major as 16-bit in-window columns, is streamed window by window;
+   each entry's x value is read from LDS and written to ``xg`` in the
+   second pass's layout.
+2. ``fs_spmv_csr`` with entry tiles and ``x_perm``: tile t's gathered values
+   lie at ``xg[tile_entry[t] .. tile_entry[t+1])``, ordered by window (the
+   order pass 1 writes them in); the tile stages them in LDS and reads entry
+   j's value through ``x_perm[j]``.  Values are read in CSR order, so the
+   tile's products and row sums are exactly those of the one-pass entry-tile
+   kernel with the same tiles: the results are bit-identical.
+
+Layout (all on the device, built once per matrix by stable sorts):
+
+* ``wcol[k]`` (uint16): in-window column of window-major entry k.  Window-
+  major order is (window, tile, CSR order).
+* runs: the entries of one tile inside one window.  ``run_off[r]``: entry k of
+  run r goes to ``xg[k + run_off[r]]``.  ``batch_run[b]`` / ``batch_mask[b]``:
+  run of entry 64 b and the entries of batch b that start a run, so the
+  gather finds an entry's run with one popcount.
+* ``x_perm[j]`` (uint16): position of CSR entry j inside its tile's slice of
+  ``xg``.
+* parts: the gather's workgroups, one per CU, each an equal range of the
+  window-major entries (a range runs through the windows it covers, staging
+  each in turn).
+
+Bytes per entry (fp32): pass 1 reads 2 (wcol) and writes 4 (xg); pass 2 reads
+4 (value) + 4 (xg) + 2 (x_perm).  16 bytes, all in streams, against 12 in the
+one-pass kernel of which 4 are random gathers.
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+from . import _lib
+from .rowsplit import RowSplitPlan, Schedule
+
+XWIN_DTYPES = (torch.float32, torch.float64)
+DEFAULT_WINDOW_BYTES = 128 * 1024
+
+
+def default_window(value_dtype, device) -> int:
+    """Columns per window: 128 KB of x, within what one workgroup can stage."""
+    item = torch.empty(0, dtype=value_dtype).element_size()
+    w = DEFAULT_WINDOW_BYTES // item
+    cap = _lib.xwin_max_window(_lib.value_code(value_dtype), device)
+    if cap > 0:
+        w = min(w, cap)
+    return max(4, w - w % 4)
+
+
+def _as_int16(t: torch.Tensor) -> torch.Tensor:
+    """int64 values in [0, 65536) stored as their 16-bit pattern."""
+    return torch.where(t >= 32768, t - 65536, t).to(torch.int16)
+
+
+def gather_parts(device, window_bytes: int) -> int:
+    """Workgroups of the gather: one per CU for each window that fits the
+    CU's LDS at once, so the whole gather runs in one round."""
+    cus, lds = 256, 160 * 1024
+    if isinstance(device, torch.device) and device.type == "cuda":
+        props = torch.cuda.get_device_properties(device)
+        cus = int(getattr(props, "multi_processor_count", cus))
+    return cus * max(1, lds // max(window_bytes, 1))
+
+
+def part_ranges(win_start, nnz: int, n_parts: int):
+    """Cut the window-major entries [0, nnz) into n_parts about equal ranges
+    (cuts on multiples of 64, the gather's batch size); a range may run
+    through several windows.  Returns host lists (part_window, part_entry):
+    the window holding each range's first entry, and the range starts plus
+    nnz at the end."""
+    import bisect
+    ws = [int(v) for v in win_start]
+    cuts = [0]
+    for i in range(1, max(int(n_parts), 1)):
+        c = -(-(nnz * i // n_parts) // 64) * 64
+        if cuts[-1] < c < nnz:
+            cuts.append(c)
+    pw = [bisect.bisect_right(ws, c) - 1 for c in cuts]
+    return pw, cuts + [nnz]
+
+
+class XWinCsr:
+    """The x-window layout of one CSR matrix, its gathered-x buffer and the
+    second pass's entry-tile plans (one per accumulation mode)."""
+
+    def __init__(self, data, indices, indptr, n_rows, n_cols, tile_nnz=2048, window=None):
+        if data.dtype not in XWIN_DTYPES:
+            raise TypeError(f"x windows support {XWIN_DTYPES}, got {data.dtype}")
+        dev = indices.device
+        nnz = int(indices.numel())
+        if nnz > 2**31 - 1:
+            raise ValueError("x windows address at most 2^31 - 1 entries")
+        self.data, self.indices, self.indptr = data, indices, indptr
+        self.n_rows, self.n_cols, self.nnz = int(n_rows), int(n_cols), nnz
+        self.device = dev
+        self.value_dtype = data.dtype
+        self.tile_nnz = int(tile_nnz)
+        W = int(window) if window else default_window(data.dtype, dev)
+        if W <= 0 or W % 4 or W > 65532:
+            raise ValueError(f"window must be a multiple of 4 in (0, 65532], got {W}")
+        self.window = W
+        self.n_windows = max(1, -(-self.n_cols // W))
+        self.schedule = Schedule(tile_nnz=self.tile_nnz, tile_mode=_lib.FS_TILE_ENTRIES)
+        self._plans = {}
+        base = RowSplitPlan(indptr, self.n_rows, data.dtype, self.schedule, _lib.FS_ACC_NATIVE)
+        self._plans[_lib.FS_ACC_NATIVE] = base
+        self.tile_entry = base.tile_entry
+        self.n_tiles = base.n_tiles
+        self._build(indices, base)
+        self.xg = torch.empty(max(nnz, 1), dtype=data.dtype, device=dev)
+        self.csr = _lib.make_csr(data, indices, indptr, self.n_rows, self.n_cols)
+
+    def _build(self, indices, base):
+        dev, nnz, W, S = self.device, self.nnz, self.window, self.n_windows
+        n_tiles = self.n_tiles
+        te = self.tile_entry
+        if nnz == 0:
+            z16 = torch.zeros(1, dtype=torch.int16, device=dev)
+            self.wcol, self.x_perm = z16, z16
+            self.batch_run = torch.zeros(1, dtype=torch.int32, device=dev)
+            self.batch_mask = torch.zeros(1, dtype=torch.int64, device=dev)
+            self.run_off = torch.zeros(1, dtype=torch.int32, device=dev)
+            self.n_runs = 0
+            self.part_window = torch.zeros(1, dtype=torch.int32, device=dev)
+            self.part_entry = torch.zeros(2, dtype=torch.int64, device=dev)
+            self.window_entry = torch.zeros(S + 1, dtype=torch.int64, device=dev)
+            self.n_parts = 0
+            self.layout = self._struct()
+            return
+        cols = indices.to(torch.int64)
+        win = cols // W
+        tile = torch.repeat_interleave(torch.arange(n_tiles, device=dev, dtype=torch.int64),
+                                       te[1:] - te[:-1])
+        # window-major order (window, tile, CSR order): entries are already in
+        # tile order, so a stable sort by window is enough
+        order1 = torch.argsort(win, stable=True)
+        self.wcol = _as_int16((cols - win * W)[order1]).contiguous()
+        del cols
+        # pass-2 layout: by (tile, window, CSR order)
+        order2 = torch.argsort(tile * S + win, stable=True)
+        pos2 = torch.empty(nnz, dtype=torch.int64, device=dev)
+        pos2[order2] = torch.arange(nnz, device=dev, dtype=torch.int64)
+        del order2
+        self.x_perm = _as_int16(pos2 - te[tile]).contiguous()
+        # runs: (window, tile) groups in window-major order
+        key1 = (win * n_tiles + tile)[order1]
+        del tile
+        is_start = torch.ones(nnz, dtype=torch.bool, device=dev)
+        is_start[1:] = key1[1:] != key1[:-1]
+        del key1
+        run_k0 = torch.nonzero(is_start, as_tuple=False).flatten()
+        self.n_runs = int(run_k0.numel())
+        dst = pos2[order1]
+        del pos2
+        self.run_off = (dst[run_k0] - run_k0).to(torch.int32).contiguous()
+        del dst
+        nb = -(-nnz // 64)
+        run_id = torch.cumsum(is_start.to(torch.int64), 0) - 1
+        self.batch_run = run_id[torch.arange(nb, device=dev, dtype=torch.int64) * 64] \
+            .to(torch.int32).contiguous()
+        del run_id
+        pad = torch.zeros(nb * 64, dtype=torch.int64, device=dev)
+        pad[:nnz] = is_start.to(torch.int64)
+        bits = torch.bitwise_left_shift(pad.view(nb, 64),
+                                         torch.arange(64, device=dev, dtype=torch.int64))
+        self.batch_mask = bits.sum(dim=1).contiguous()  # distinct bits: the sum is the OR
+        del pad, bits, is_start
+        # window ranges in window-major order -> gather parts
+        win_sorted = win[order1]
+        ws = torch.searchsorted(win_sorted, torch.arange(S + 1, device=dev, dtype=torch.int64))
+        del win_sorted, win, order1
+        self.window_entry = ws.contiguous()
+        item = torch.empty(0, dtype=self.value_dtype).element_size()
+        pw, pe = part_ranges(ws.cpu().tolist(), nnz, gather_parts(dev, W * item))
+        self.part_window = torch.tensor(pw, dtype=torch.int32, device=dev)
+        self.part_entry = torch.tensor(pe, dtype=torch.int64, device=dev)
+        self.n_parts = len(pw)
+        self.layout = self._struct()
+
+    def _struct(self) -> _lib.FsXwin:
+        return _lib.FsXwin(value_type=_lib.value_code(self.value_dtype), window=self.window,
+                           n_cols=self.n_cols, nnz=self.nnz, n_runs=self.n_runs,
+                           n_parts=self.n_parts, wcol=_lib.ptr(self.wcol),
+                           batch_run=_lib.ptr(self.batch_run),
+                           batch_mask=_lib.ptr(self.batch_mask), run_off=_lib.ptr(self.run_off),
+                           part_window=_lib.ptr(self.part_window),
+                           part_entry=_lib.ptr(self.part_entry),
+                           window_entry=_lib.ptr(self.window_entry))
+
+    def plan(self, accumulate: int) -> RowSplitPlan:
+        p = self._plans.get(int(accumulate))
+        if p is None:
+            p = RowSplitPlan(self.indptr, self.n_rows, self.value_dtype, self.schedule,
+                             accumulate)
+            self._plans[int(accumulate)] = p
+        if p.x_perm is None:
+            p.set_x_perm(self.x_perm)
+        return p
+
+    def run(self, x, y, alpha, beta, accumulate, device):
+        """y = alpha A x + beta y (x dense, n_cols values)."""
+        if self.n_rows == 0:
+            return
+        if self.nnz > 0:
+            _lib.xwin_gather(self.layout, x, self.xg, device)
+        _lib.spmv(self.csr, self.xg, y, alpha, beta, self.plan(accumulate).struct(1), device)
+
+    def footprint_bytes(self) -> int:
+        """Device bytes this layout adds to the matrix."""
+        t = (self.wcol, self.x_perm, self.batch_run, self.batch_mask, self.run_off,
+             self.part_window, self.part_entry, self.window_entry, self.xg)
+        return int(sum(a.numel() * a.element_size() for a in t))
+
+
+def window_count(n_cols: int, window: int) -> int:
+    return max(1, math.ceil(n_cols / max(window, 1)))

@@ -124,6 +124,7 @@ typedef struct fs_plan {
   const int64_t *group_tiles;    /* [n_groups + 1] tile range of a group   */
   void *row_sums;                /* FS_TILE_ENTRIES: if set, raw row sums  */
   const int64_t *tile_entry;     /* FS_TILE_ENTRIES: [n_tiles + 1]         */
+  const uint16_t *x_perm;        /* FS_TILE_ENTRIES: if set, x is gathered */
 } fs_plan;
 
 /*
@@ -154,6 +155,13 @@ typedef struct fs_plan {
  *                    sum is stored to row_sums[r] in the sum type (fp64; two
  *                    fp64 for complex), times its row scale for
  *                    FS_STORE_F16_ROWSCALE, for fs_sliced_combine to add up.
+ *                    With x_perm set (fp32 / fp64 values, no row_sums), x is
+ *                    not the dense vector but the nnz x values already
+ *                    gathered per entry by fs_xwin_gather: tile t's values
+ *                    lie at x[tile_entry[t] .. tile_entry[t+1]), in the
+ *                    order x_perm gives -- entry j of tile t takes
+ *                    x[tile_entry[t] + x_perm[j]] -- and col_idx is not read
+ *                    (tile_nnz 1024, 2048 or 4096).
  */
 enum { FS_TILE_ROWS = 0, FS_TILE_ENTRIES = 1 };
 
@@ -238,6 +246,44 @@ typedef struct fs_slices {
 
 int fs_sliced_combine(const fs_slices *sl, const void *z, void *y, double alpha, double beta,
                       void *stream);
+
+/* ---------------------------------------------------- x-window SpMV */
+/*
+ * No counterpart in the reference (whose SpMV gathers x in place,
+ * spmv_csr.py:158-184).  First pass of a two-pass SpMV for x far larger than
+ * the L2: the columns are cut into windows of `window` columns, and a second
+ * copy of the column indices is stored window-major (window by window; inside
+ * a window by entry tile, then in CSR order) as 16-bit in-window columns.  A
+ * run is the entries of one entry tile inside one window.  fs_xwin_gather
+ * stages each window of x in one workgroup's LDS and writes every entry's x
+ * value to xg: entry k (window-major position) of run r goes to
+ * xg[k + run_off[r]].  The second pass is fs_spmv_csr with entry tiles and
+ * fs_plan.x_perm.
+ */
+typedef struct fs_xwin {
+  int32_t value_type;           /* of x and xg: FS_F32 or FS_F64                */
+  int32_t window;               /* columns per window: a multiple of 4, at most
+                                   fs_xwin_max_window(value_type)               */
+  int64_t n_cols;
+  int64_t nnz;
+  int64_t n_runs;
+  int64_t n_parts;              /* workgroups of the gather                     */
+  const uint16_t *wcol;         /* [nnz] column - window * (column / window)    */
+  const int32_t *batch_run;     /* [ceil(nnz / 64)] run of entry 64 b           */
+  const uint64_t *batch_mask;   /* [ceil(nnz / 64)] bit i (i >= 1) set when
+                                   entry 64 b + i starts a run                  */
+  const int32_t *run_off;       /* [n_runs]                                     */
+  const int32_t *part_window;   /* [n_parts] window holding part p's first entry */
+  const int64_t *part_entry;    /* [n_parts + 1] part p = window-major entries
+                                   [part_entry[p], part_entry[p + 1]); a part
+                                   may run through several windows             */
+  const int64_t *window_entry;  /* [ceil(n_cols / window) + 1] first
+                                   window-major entry of each window          */
+} fs_xwin;
+
+/* Largest window (columns) one workgroup can stage on the current device, or -1. */
+int64_t fs_xwin_max_window(int32_t value_type);
+int fs_xwin_gather(const fs_xwin *L, const void *x, void *xg, void *stream);
 
 /* Bytes of `partials` needed for n_chunks chunks (n_dense = 1 for SpMV). */
 int64_t fs_partials_bytes(int32_t value_type, int32_t accumulate, int64_t n_chunks,

@@ -34,7 +34,7 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     classes = {"fs_csr": _lib.FsCsr, "fs_plan": _lib.FsPlan, "fs_batch_entry": _lib.FsBatchEntry,
                "fs_mlp": _lib.FsMlp, "fs_slices": _lib.FsSlices,
-               "fs_mtx_info": _lib.FsMtxInfo}
+               "fs_mtx_info": _lib.FsMtxInfo, "fs_xwin": _lib.FsXwin}
     fields = {st: [f for f, _ in cls._fields_] for st, cls in classes.items()}
     fields["fs_batch_item"] = ["matrix", "kind", "row_begin", "row_end"]
     lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){"]

@@ -1,0 +1,121 @@
+"""GPU parity of the x-window SpMV (x gathered from LDS windows, then entry
+tiles over the gathered values).
+
+Both passes keep the entry-tile kernel's products and summation order, so the
+x-window result must equal the one-pass entry-tile result with the same tile
+size bit for bit, for every accumulation mode; and both must meet the
+oracle's bars (fp32 default path: <= 1 ulp of fl32(y64) and >= 99.9 %
+bit-identical rows; fp32 native: 1e-5 of sum|a x| per row; fp64: 1e-12).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from flagsparse_public_amd import flagsparse_spmv_csr, prepare_spmv_csr, synthetic
+from flagsparse_public_amd.rowsplit import Schedule
+from tests._util import host64, row_rel_err, to_dev, ulp_distance_f32
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def _matrix(kind, dtype):
+    npd = np.float64 if dtype == torch.float64 else np.float32
+    if kind == "powerlaw":
+        ip, cols, vals = synthetic.powerlaw_csr(200_000, 21, alpha=1.9, max_len=60_000, dtype=npd)
+    elif kind == "banded":
+        ip, cols, vals = synthetic.banded_csr(150_000, 22, 17, dtype=npd)
+    else:  # poisson rows with a run of empty rows and columns past the last window
+        ip, cols, vals = synthetic.poisson_csr(90_000, 14.0, 23, dtype=npd, n_cols=170_003)
+        lens = np.diff(ip)
+        lens[20_000:26_000] = 0
+        keep = np.ones(cols.size, bool)
+        keep[ip[20_000]:ip[26_000]] = False
+        ip = np.zeros_like(ip)
+        np.cumsum(lens, out=ip[1:])
+        cols, vals = cols[keep], vals[keep]
+    n_cols = 170_003 if kind == "poisson" else ip.size - 1
+    return ip, cols, vals, n_cols
+
+
+def _oracle(orc, ip, cols, vals, x):
+    return orc.spmv(ip, cols, vals, x), orc.spmv_abs(ip, cols, vals, x)
+
+
+def _check(y, y64, cond, dtype, use_opt):
+    y = host64(y)
+    if dtype == torch.float32 and not use_opt:
+        ulps = ulp_distance_f32(y.astype(np.float32), y64.astype(np.float32))
+        assert ulps.max() <= 1 and (ulps == 0).mean() >= 0.999
+    else:
+        tol = 1e-5 if dtype == torch.float32 else 1e-12
+        assert row_rel_err(y, y64, cond).max() <= tol
+
+
+CASES = [(torch.float32, True, 1024), (torch.float32, True, 2048), (torch.float32, True, 4096),
+         (torch.float32, False, 2048), (torch.float32, False, 4096),
+         (torch.float64, True, 1024), (torch.float64, True, 2048)]
+
+
+@pytest.mark.parametrize("dtype,use_opt,T", CASES)
+@pytest.mark.parametrize("kind", ["powerlaw", "banded", "poisson"])
+@pytest.mark.parametrize("window", [-1, 4096])
+def test_xwin_equals_entry_tiles(orc, kind, dtype, use_opt, T, window):
+    ip, cols, vals, n_cols = _matrix(kind, dtype)
+    n = ip.size - 1
+    print(f"[xwin] {kind} n={n} nnz={vals.size} {dtype} opt={use_opt} T={T} W={window}",
+          flush=True)
+    x = synthetic.dense_uniform(5, n_cols, dtype=vals.dtype)
+    args = [to_dev(vals), to_dev(cols), to_dev(ip)]
+    ref = prepare_spmv_csr(*args, (n, n_cols), schedule=Schedule(tile_nnz=T, tile_mode=1))
+    xw = prepare_spmv_csr(*args, (n, n_cols),
+                          schedule=Schedule(tile_nnz=T, tile_mode=1, x_window=window))
+    dx = to_dev(x)
+    y_ref = flagsparse_spmv_csr(x=dx, prepared=ref, use_opt=use_opt)
+    y = flagsparse_spmv_csr(x=dx, prepared=xw, use_opt=use_opt)
+    assert torch.equal(y, y_ref)
+    y2 = flagsparse_spmv_csr(x=dx, prepared=xw, use_opt=use_opt)  # reuses the layout
+    assert torch.equal(y2, y)
+    y64, cond = _oracle(orc, ip, cols, vals, x)
+    _check(y, y64, cond, dtype, use_opt)
+
+
+def test_xwin_alpha_beta_int64_and_misaligned_x(orc):
+    ip, cols, vals, n_cols = _matrix("powerlaw", torch.float32)
+    n = ip.size - 1
+    x = synthetic.dense_uniform(6, n_cols + 1)
+    dxx = to_dev(x)[1:]  # 4-byte offset: the window staging takes its scalar path
+    y0 = synthetic.dense_uniform(7, n)
+    for idx_dtype in (np.int32, np.int64):
+        args = [to_dev(vals), to_dev(cols.astype(idx_dtype)), to_dev(ip.astype(idx_dtype))]
+        p = prepare_spmv_csr(*args, (n, n_cols),
+                             schedule=Schedule(tile_nnz=2048, tile_mode=1, x_window=-1))
+        out = to_dev(y0)
+        res = flagsparse_spmv_csr(x=dxx, prepared=p, use_opt=True, out=out, alpha=0.5, beta=-2.0)
+        assert res is out
+        y64, cond = _oracle(orc, ip, cols, vals, x[1:])
+        want = 0.5 * y64 - 2.0 * y0.astype(np.float64)
+        err = row_rel_err(host64(res), want, 0.5 * cond + 2.0 * np.abs(y0))
+        assert err.max() <= 1e-5
+
+
+@pytest.mark.slow
+def test_xwin_powerlaw_4m_full_size(orc):
+    """The 4M-row power-law matrix (48.6M entries, x of 16 MB): the x-window
+    result equals the one-pass entry tiles and meets the oracle's bars."""
+    ip, cols, vals = synthetic.spmv_powerlaw_4m()
+    n = ip.size - 1
+    print(f"[xwin] powerlaw_4m nnz={vals.size}", flush=True)
+    x = synthetic.dense_uniform(2, n)
+    args = [to_dev(vals), to_dev(cols), to_dev(ip)]
+    y64, cond = _oracle(orc, ip, cols, vals, x)
+    dx = to_dev(x)
+    ref = prepare_spmv_csr(*args, (n, n), schedule=Schedule(tile_nnz=2048, tile_mode=1))
+    xw = prepare_spmv_csr(*args, (n, n),
+                          schedule=Schedule(tile_nnz=2048, tile_mode=1, x_window=-1))
+    for use_opt in (True, False):
+        y = flagsparse_spmv_csr(x=dx, prepared=xw, use_opt=use_opt)
+        assert torch.equal(y, flagsparse_spmv_csr(x=dx, prepared=ref, use_opt=use_opt))
+        _check(y, y64, cond, torch.float32, use_opt)

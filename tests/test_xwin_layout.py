@@ -1,0 +1,116 @@
+"""The x-window layout, checked on the CPU by replaying both passes in numpy.
+
+Pass 1 as the gather kernel runs it: entry k (window-major) finds its run from
+its batch's first run plus the popcount of the batch's run-start bits below
+it, and writes x[window * W + wcol[k]] to xg[k + run_off[run]].  Pass 2 as the
+entry kernel reads it: CSR entry j of tile t takes xg[tile_entry[t] +
+x_perm[j]].  With distinct x values, every entry must read exactly
+x[col[j]], the destinations must be a permutation, and each gather part must
+lie inside one window.  Then sum(val * xg) per row equals the oracle's SpMV.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from flagsparse_public_amd import synthetic
+from flagsparse_public_amd.xwin import XWinCsr, part_ranges
+
+
+def _replay(xw, x):
+    nnz = xw.nnz
+    wcol = xw.wcol.numpy().view(np.uint16).astype(np.int64)
+    br = xw.batch_run.numpy().astype(np.int64)
+    bm = xw.batch_mask.numpy().view(np.uint64)
+    off = xw.run_off.numpy().astype(np.int64)
+    pw = xw.part_window.numpy().astype(np.int64)
+    pe = xw.part_entry.numpy()
+    ws = xw.window_entry.numpy()
+    assert pe[0] == 0 and pe[-1] == nnz and np.all(np.diff(pe) > 0)
+    assert ws[0] == 0 and ws[-1] == nnz and np.all(np.diff(ws) >= 0)
+    assert np.all((ws[pw] <= pe[:-1]) & (pe[:-1] < ws[pw + 1]))  # first window of each part
+    k = np.arange(nnz, dtype=np.int64)
+    lane = (k % 64).astype(np.uint64)
+    below = (np.uint64(2) << lane) - np.uint64(2)  # lane 63: 2 << 63 wraps to 0
+    r = br[k // 64] + np.bitwise_count(bm[k // 64] & below).astype(np.int64)
+    assert r.min() >= 0 and r.max() < xw.n_runs
+    dst = k + off[r]
+    assert np.array_equal(np.sort(dst), k), "destinations are not a permutation"
+    win = np.searchsorted(ws, k, side="right") - 1  # the window of each entry
+    xg = np.empty(nnz, dtype=x.dtype)
+    xg[dst] = x[win * xw.window + wcol]
+    return xg
+
+
+def _check(ip, cols, vals, T, W, orc):
+    n_rows = ip.size - 1
+    n_cols = int(cols.max()) + 1 if cols.size else 1
+    n_cols = max(n_cols, 7 * W // 2 + 3)  # a last window narrower than W
+    xw = XWinCsr(torch.from_numpy(vals), torch.from_numpy(cols), torch.from_numpy(ip),
+                 n_rows, n_cols, tile_nnz=T, window=W)
+    x = np.arange(n_cols, dtype=np.float64) + 0.25  # distinct
+    xg = _replay(xw, x) if xw.nnz else np.zeros(0)
+    te = xw.tile_entry.numpy()
+    perm = xw.x_perm.numpy().view(np.uint16).astype(np.int64)
+    tile = np.repeat(np.arange(xw.n_tiles), np.diff(te))
+    assert perm.size == xw.nnz or xw.nnz == 0
+    if xw.nnz:
+        assert perm.max() < T
+        got = xg[te[tile] + perm]
+        assert np.array_equal(got, x[cols.astype(np.int64)])
+        # each tile's x_perm is a permutation of its slots
+        for t in range(0, xw.n_tiles, max(1, xw.n_tiles // 7)):
+            p = np.sort(perm[te[t]:te[t + 1]])
+            assert np.array_equal(p, np.arange(te[t + 1] - te[t]))
+    # the two passes give the oracle's SpMV
+    lens = np.diff(ip)
+    rows = np.repeat(np.arange(n_rows), lens)
+    y = np.zeros(n_rows)
+    if xw.nnz:
+        np.add.at(y, rows, vals.astype(np.float64) * xg[te[tile] + perm])
+    y64 = orc.spmv(ip, cols, vals.astype(np.float64), x)
+    assert np.allclose(y, y64, rtol=1e-12, atol=1e-9)
+    return xw
+
+
+@pytest.mark.parametrize("T", [1024, 2048])
+@pytest.mark.parametrize("W", [64, 1000, 4096])
+def test_powerlaw_layout(orc, T, W):
+    ip, cols, vals = synthetic.powerlaw_csr(6000, 5, alpha=1.8, max_len=5000)
+    _check(ip, cols, vals, T, W, orc)
+
+
+def test_banded_and_empty_rows(orc):
+    ip, cols, vals = synthetic.banded_csr(5000, 3, 9)
+    lens = np.diff(ip)
+    lens[::3] = 0  # every third row empty
+    keep = np.repeat(np.arange(lens.size), np.diff(ip)) % 3 != 0
+    ip2 = np.zeros_like(ip)
+    np.cumsum(lens, out=ip2[1:])
+    _check(ip2, cols[keep], vals[keep], 2048, 256, orc)
+
+
+def test_int64_indices_and_single_window(orc):
+    ip, cols, vals = synthetic.poisson_csr(3000, 9.0, 4, index_dtype=np.int64)
+    _check(ip.astype(np.int64), cols, vals, 1024, 4096, orc)
+    _check(ip.astype(np.int64), cols, vals, 1024, 4, orc)  # windows of four columns
+
+
+def test_empty_matrix():
+    ip = np.zeros(11, np.int32)
+    xw = XWinCsr(torch.zeros(0), torch.zeros(0, dtype=torch.int32), torch.from_numpy(ip), 10,
+                 100, window=64)
+    assert xw.nnz == 0 and xw.n_parts == 0
+
+
+def test_part_ranges_cover_windows():
+    ws = [0, 10, 10, 50_000, 50_001, 200_000]
+    pw, pe = part_ranges(ws, 200_000, 16)
+    assert pe[0] == 0 and pe[-1] == 200_000 and len(pe) == 17
+    assert all(c % 64 == 0 for c in pe[:-1])  # cuts on batch boundaries
+    assert max(np.diff(pe)) - min(np.diff(pe)) <= 64  # equal parts
+    for p, w in enumerate(pw):
+        assert ws[w] <= pe[p] < ws[w + 1]  # the window holding the first entry
+    assert 1 not in pw  # the empty window holds no entry
+    pw, pe = part_ranges([0, 100], 100, 16)  # fewer batches than parts
+    assert pe == [0, 64, 100] and pw == [0, 0]
