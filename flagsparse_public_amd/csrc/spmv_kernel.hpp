@@ -467,11 +467,20 @@ void spmv_launch_tiles(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream
 // path; results depend on T only through where a long row is cut.
 
 constexpr int kSegRows = 1024;  // row boundaries staged in LDS per pass
+// Rows longer than kThreadRowMax inside one tile: at most tile_nnz / 33 (124
+// for the largest tile, 4096 entries), listed in LDS for the wavefronts.
+constexpr int kLongRowsMax = 128;
+// int slots after the products: [0, kSegRows] row boundaries, then the
+// straddle flag, the long-row count, and from kLongList the long-row list
+// (a multiple of 4 slots, so what follows stays 16-byte aligned).
+constexpr int kStraddleSlot = kSegRows + 1;
+constexpr int kLongCountSlot = kSegRows + 2;
+constexpr int kLongList = kSegRows + 4;
+constexpr int kBndSlots = kLongList + kLongRowsMax;
 
 template <typename C> __host__ __device__ inline size_t entry_lds_bytes(int64_t tile_nnz) {
   using S = typename TileSum<C>::T;
-  return sizeof(C) * (size_t)tile_nnz + sizeof(int32_t) * (kSegRows + 4) +
-         sizeof(S) * kWavesPerBlock;
+  return sizeof(C) * (size_t)tile_nnz + sizeof(int32_t) * kBndSlots + sizeof(S) * kWavesPerBlock;
 }
 
 // Gathered-x form (fs_plan.x_perm): the tile's gathered x values (type V)
@@ -687,8 +696,13 @@ __device__ __forceinline__ void entry_stage(const EntryTile<I, C, E>& f, const C
     if (i <= f.nb0) {
       const int64_t v = f.pv[k] - f.e0;
       bnd[i] = (int)(v < 0 ? 0 : (v > f.len ? f.len : v));
+      // whether the last owned row runs past the tile, from the pointer
+      // already loaded (-1: the owned rows exceed one batch, decided later)
+      if (i == f.nb0)
+        bnd[kStraddleSlot] = f.r1 - f.r0 > kSegRows ? -1 : (f.r1 > f.r0 && v > f.len ? 1 : 0);
     }
   }
+  if (tid == 0) bnd[kLongCountSlot] = 0;
 }
 
 // After the LDS stage and a barrier: the head carry (entries of a row owned
@@ -701,7 +715,8 @@ __device__ __forceinline__ void entry_rows(const SpmvParams<VS, V, I, P>& p,
   const int tid = threadIdx.x, w = tid / kWave, lane = tid % kWave;
   const int64_t t = f.t, e0 = f.e0, r0 = f.r0, r1 = f.r1;
   const int len = f.len;
-  const bool straddle = r1 > r0 && (int64_t)p.ptr[r1] > e0 + len;
+  const int sflag = bnd[kStraddleSlot];
+  const bool straddle = sflag >= 0 ? sflag == 1 : (r1 > r0 && (int64_t)p.ptr[r1] > e0 + len);
   // head: entries [0, bnd[0]) belong to a row owned by an earlier tile
   const int head = bnd[0];
   if (head > 0 && head <= 4 * kWave) {
@@ -735,19 +750,29 @@ __device__ __forceinline__ void entry_rows(const SpmvParams<VS, V, I, P>& p,
         const int64_t v = (int64_t)p.ptr[rb + i] - e0;
         bnd[i] = (int)(v < 0 ? 0 : (v > len ? len : v));
       }
+      if (tid == 0) bnd[kLongCountSlot] = 0;
       __syncthreads();
     }
+    // short rows: a thread each; longer rows are listed for the wavefronts
+    // (at most len / (kThreadRowMax + 1) < kLongRowsMax of them)
     for (int i = tid; i < nb; i += kBlock) {
       const int s = bnd[i], e = bnd[i + 1];
-      if (e - s > kThreadRowMax) continue;
+      if (e - s > kThreadRowMax) {
+        const int slot = atomicAdd(&bnd[kLongCountSlot], 1);
+        if (slot < kLongRowsMax) bnd[kLongList + slot] = i;
+        continue;
+      }
       S acc = zero<S>();
       for (int j = s; j < e; ++j) acc = add(acc, widen(prod[j]));
       const int64_t row = rb + i;
       entry_finish<A>(p, row, acc, straddle && row == r1 - 1, carries, t);
     }
-    for (int i = w; i < nb; i += kWavesPerBlock) {
+    __syncthreads();
+    int n_long = bnd[kLongCountSlot];
+    n_long = n_long < kLongRowsMax ? n_long : kLongRowsMax;
+    for (int q = w; q < n_long; q += kWavesPerBlock) {
+      const int i = bnd[kLongList + q];
       const int s = bnd[i], e = bnd[i + 1];
-      if (e - s <= kThreadRowMax) continue;
       Plain<S> acc;
       acc.init();
       for (int j = s + lane; j < e; j += kWave) acc.add_value(widen(prod[j]));
@@ -767,7 +792,7 @@ template <typename C, typename S, int T> struct EntryLds {
   __device__ __forceinline__ explicit EntryLds(char* smem)
       : prod(reinterpret_cast<C*>(smem)),
         bnd(reinterpret_cast<int32_t*>(smem + sizeof(C) * T)),
-        red(reinterpret_cast<S*>(smem + sizeof(C) * T + sizeof(int32_t) * (kSegRows + 4))) {}
+        red(reinterpret_cast<S*>(smem + sizeof(C) * T + sizeof(int32_t) * kBndSlots)) {}
 };
 
 // One tile per block (block b takes tile b, or with block groups the b / G-th

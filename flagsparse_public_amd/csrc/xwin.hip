@@ -26,7 +26,7 @@ namespace fs {
 
 constexpr int kMaxXwDevices = 64;
 constexpr int kXwBlock = 1024;   // threads per workgroup (16 wavefronts)
-constexpr int kXwRound = 8;      // 64-entry batches per wavefront in flight
+constexpr int kXwRound = 16;     // 64-entry batches per wavefront in flight
 
 template <typename V> struct XwParams {
   const V* x;
@@ -95,10 +95,15 @@ __global__ __launch_bounds__(kXwBlock) void xwin_gather_kernel(XwParams<V> p) {
     const int64_t b_first = k0 / kWave, b_end = (k1 + kWave - 1) / kWave;
     for (int64_t b0 = b_first + (int64_t)wave * kXwRound; b0 < b_end;
          b0 += (int64_t)kWaves * kXwRound) {
+      // The round's batch metadata with one vector load (lane i < kXwRound
+      // takes batch b0 + i), broadcast per batch with readlane; then every
+      // column load.  Indices are clamped into the range, and every load is
+      // issued before any is used.
+      int64_t bl = b0 + (lane < kXwRound ? lane : 0);
+      bl = bl < b_end ? bl : b_end - 1;
+      const int32_t brv = p.batch_run[bl];
+      const uint64_t bmv = p.batch_mask[bl];
       uint16_t wc[kXwRound];
-      int32_t br[kXwRound];
-      uint64_t bm[kXwRound];
-      // every load is issued (indices clamped into the range) before any is used
 #pragma unroll
       for (int i = 0; i < kXwRound; ++i) {
         int64_t b = b0 + i;
@@ -106,13 +111,15 @@ __global__ __launch_bounds__(kXwBlock) void xwin_gather_kernel(XwParams<V> p) {
         int64_t k = b * kWave + lane;
         k = k < k0 ? k0 : (k >= k1 ? k1 - 1 : k);
         wc[i] = p.wcol[k];
-        br[i] = p.batch_run[b];
-        bm[i] = p.batch_mask[b];
       }
       int32_t off[kXwRound];
 #pragma unroll
       for (int i = 0; i < kXwRound; ++i) {
-        int64_t r = (int64_t)br[i] + __builtin_popcountll(bm[i] & below);
+        const int32_t br = __builtin_amdgcn_readlane(brv, i);
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)bmv, i);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(bmv >> 32), i);
+        const uint64_t bm = ((uint64_t)hi << 32) | lo;
+        int64_t r = (int64_t)br + __builtin_popcountll(bm & below);
         r = r < 0 ? 0 : (r > rmax ? rmax : r);
         off[i] = p.run_off[r];
       }
