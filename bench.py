@@ -30,7 +30,9 @@ reference (SciPy's CSR product in float64):
     graph-replayed persistent launch against the 1,024 separate launches;
   * selection: on the 4M-row SpMV and the N = 64 SpMM, the learned choice
     against exhaustive autotuning (every variant timed) and the plain
-    default-tile kernel.
+    default-tile kernel;
+  * unprepared_calls: flagsparse_spmv_csr(data, indices, indptr, x, shape)
+    repeated on the same tensors (the reference's default call pattern).
 --quick skips them.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--path opt|default] [--dtype f32|f64]
@@ -57,10 +59,10 @@ def spmv_bytes(n_rows, nnz, s_val=4, s_idx=4, s_ptr=4, s_x=4, s_y=4):
     return (n_rows + 1) * s_ptr + nnz * (s_idx + s_val + s_x) + n_rows * s_y
 
 
-def _traffic_from_profile():
+def _traffic_from_profile(name="spmv_powerlaw_4m_pmc.txt"):
     """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE),
     scaled by the copy-kernel calibration, from the committed summary."""
-    path = os.path.join(ROOT, "profiles", "spmv_powerlaw_4m_pmc.txt")
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -536,6 +538,53 @@ def sub_selection(d_spmv, spmv_prepared, steps):
     return out
 
 
+def sub_unprepared(d_spmv, prepared_us, reps=20):
+    """Unprepared calls flagsparse_spmv_csr(data, indices, indptr, x, shape)
+    -- the reference's default call pattern -- on the 4M-row matrix and on
+    the 16K-row one: wall time per call (each call runs the validation pass
+    and its host read, as the reference's does), beside the prepared step and
+    the validation pass alone.  The first call prepares; later calls on the
+    same tensors reuse it."""
+    import numpy as np
+    import torch
+    from flagsparse_public_amd import flagsparse_spmv_csr, prepare_spmv_csr, synthetic
+    from flagsparse_public_amd.sparse_operations.spmv_csr import (_UNPREPARED,
+                                                                  _prepare_spmv_csr_matrix)
+    dev = torch.device("cuda", 0)
+    out = {}
+
+    def wall(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e6 / k
+
+    cases = [("spmv_powerlaw_4m", d_spmv, prepared_us)]
+    ip, cols, vals = synthetic.spmv_16k()
+    n1 = ip.size - 1
+    d1 = tuple(torch.from_numpy(a).to(dev) for a in (vals, cols, ip))
+    x1 = torch.from_numpy(synthetic.dense_uniform(11, n1)).to(dev)
+    p1 = prepare_spmv_csr(*d1, (n1, n1))
+    us1 = _event_time(lambda: flagsparse_spmv_csr(x=x1, prepared=p1, use_opt=True), 100,
+                      torch.cuda.current_stream(dev))
+    cases.append(("spmv_16k", d1 + (x1,), us1))
+    for name, (vals_d, cols_d, ip_d, x_d), step_us in cases:
+        n = int(ip_d.numel()) - 1
+        _UNPREPARED.clear()
+        call = lambda: flagsparse_spmv_csr(vals_d, cols_d, ip_d, x_d, (n, n), use_opt=True)
+        first = wall(call, 1)
+        again = wall(call, reps)
+        valid = wall(lambda: _prepare_spmv_csr_matrix(vals_d, cols_d, ip_d, (n, n)), reps)
+        out[name] = {"first_call_us": round(first, 1), "repeat_call_us": round(again, 1),
+                     "prepared_step_us": round(step_us, 2), "validation_pass_us": round(valid, 1),
+                     "repeat_over_step_plus_validation": round(again / (step_us + valid), 3)}
+    _UNPREPARED.clear()
+    _log(f"unprepared calls: {out}")
+    return out
+
+
 def _selection_record(times, learned, default, rule, how):
     best = int(times.argmin())
     rec = {"variants_timed": int(times.size), "exhaustive_best": best,
@@ -655,12 +704,35 @@ def main(argv=None):
     _log(f"{args.steps} steps: events {ev_ms:.3f} ms ({per_launch_ms * 1e3:.2f} us/launch), "
          f"wall {wall_ms:.3f} ms, {gbs:.1f} GB/s")
     sch = prepared.schedule
-    # the dominant kernel alone: the compact-row SpMV of the sliced layout, or
-    # the whole SpMV when the matrix is not sliced
+    # the dominant kernel alone: the gather pass of the x-window layout, the
+    # compact-row SpMV of the sliced layout, or the whole SpMV otherwise
     from flagsparse_public_amd import _lib
     sliced = prepared._plans.get(("sliced", sch.key(), _lib.FS_ACC_NATIVE if use_opt
                                   else _lib.FS_ACC_WIDE))
-    if sliced is not None:
+    xw = prepared._plans.get(("xwin", sch.key()))
+    kernels = None
+    if xw is not None:
+        acc = _lib.FS_ACC_NATIVE if use_opt else _lib.FS_ACC_WIDE
+        pst = xw.plan(acc).struct(1)
+        yk = torch.empty(n, dtype=d_vals.dtype, device=dev)
+
+        def kernel():
+            _lib.xwin_gather(xw.layout, d_x, xw.xg, dev)
+        n_batches = -(-nnz // 64)
+        # gather pass: wcol (2 B) read and xg written per entry, x read once,
+        # 4 B per run and 12 B per 64-entry batch of run records
+        kbytes = nnz * (2 + s_v) + n * s_v + xw.n_runs * 4 + n_batches * 12
+        kname = "xwin_gather_kernel (x windows of %d columns in LDS)" % xw.window
+        # pass 2 (entry tiles over the gathered x) and its carry fix-up, timed alone
+        p2_bytes = (n + 1) * 4 + nnz * (2 * s_v + 2) + n * s_v
+        p2_us = _event_time(lambda: _lib.spmv(xw.csr, xw.xg, yk, 1.0, 0.0, pst, dev),
+                            args.steps, stream)
+        kernels = [{"kernel": "spmv_entry_kernel gathered-x form + carry fix-up",
+                    "us": round(p2_us, 3), "algorithmic_bytes": p2_bytes,
+                    "GB/s": round(p2_bytes / p2_us / 1e3, 1),
+                    "frac": round(p2_bytes / p2_us / 1e3 / HBM_PEAK_GBS, 4)}]
+        nc = None
+    elif sliced is not None:
         kplan = sliced.plan.struct(1)
 
         def kernel():
@@ -686,6 +758,10 @@ def main(argv=None):
     kernel_us = k0.elapsed_time(k1) * 1e3 / args.steps
     kgbs = kbytes / (kernel_us * 1e-6) / 1e9
     _log(f"dominant kernel: {kernel_us:.2f} us/launch, {kgbs:.1f} GB/s over {kbytes} bytes")
+    if kernels is not None:
+        kernels.insert(0, {"kernel": kname, "us": round(kernel_us, 3), "algorithmic_bytes": kbytes,
+                           "GB/s": round(kgbs, 1), "frac": round(kgbs / HBM_PEAK_GBS, 4)})
+        _log(f"kernels: {kernels}")
     result = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -707,6 +783,8 @@ def main(argv=None):
                          "tile_nnz": sch.tile_nnz, "x_slice_cols": sch.x_slice_cols,
                          "n_slices": sliced.n_slices if sliced is not None else 1,
                          "compact_rows": nc,
+                         "x_window": xw.window if xw is not None else 0,
+                         "x_windows": xw.n_windows if xw is not None else 0,
                          "chosen_by": prepared.schedule_record["chosen_by"]},
             "gflops": round(2.0 * nnz / (per_launch_ms * 1e-3) / 1e9, 2),
             "compulsory_bytes": (n + 1) * 4 + nnz * (4 + s_v) + n * s_v + n * s_v,
@@ -718,13 +796,17 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(kgbs / HBM_PEAK_GBS, 4),
-            "traffic": _traffic_from_profile() if args.dtype == "f32" else None,
+            "traffic": (_traffic_from_profile("spmv_powerlaw_4m_xwin_pmc.txt" if xw is not None
+                                              else "spmv_powerlaw_4m_pmc.txt")
+                        if args.dtype == "f32" else None),
             "algorithmic_bytes_per_launch": kbytes,
             "avg_launch_us": round(kernel_us, 3),
             "step_us": round(per_launch_ms * 1e3, 3),
             "step_frac": round(gbs / HBM_PEAK_GBS, 4),
         },
     }
+    if kernels is not None:
+        result["roofline"]["kernels"] = kernels
     if not args.no_vendor:
         result["vendor_baseline"] = vendor_spmv(d_vals, d_cols, d_ip, d_x, n, nbytes, args.steps)
     modes = None
@@ -754,6 +836,9 @@ def main(argv=None):
         t_sub = time.time()
         work["selection"] = sub_selection((d_vals, d_cols, d_ip, d_x), prepared, sub_steps)
         prepared._plans.clear()  # the variants' layouts
+        torch.cuda.empty_cache()
+        work["unprepared_calls"] = sub_unprepared((d_vals, d_cols, d_ip, d_x),
+                                                  per_launch_ms * 1e3)
         torch.cuda.empty_cache()
         work["spmv_powerlaw_4m_f64"] = sub_spmv_f64(sub_steps)
         torch.cuda.empty_cache()

@@ -114,6 +114,8 @@ class FsSlices(ctypes.Structure):
         ("rows_per_block", ctypes.c_int32),
         ("z_type", ctypes.c_int32),
         ("row_mask", ctypes.c_void_p),
+        ("fp32_sums", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

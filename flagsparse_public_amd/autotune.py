@@ -24,12 +24,15 @@ import torch
 
 from .rowsplit import Schedule
 
-VARIANTS_VERSION = 4
+# Version of each operation's variant list: a timing database or a model
+# recorded for another list is not used.
+VARIANTS_VERSION = {"spmv": 5, "spmm64": 4}
 
 # SpMV (fp32, native accumulation) variants.  0-5: entry tiles (fixed runs of
 # entries), 3-5 with the columns cut into L2-sized slices when x is larger
 # than one slice; 6-8: row tiles with chunked long rows; 9-15: one lane group
-# per row (S lanes), long rows chunked.
+# per row (S lanes), long rows chunked; 16-17: entry tiles over x gathered
+# from LDS-staged windows (128 KB / 160 KB of x per window).
 SPMV_VARIANTS = (
     Schedule(tile_nnz=1024, tile_mode=1),
     Schedule(tile_nnz=2048, tile_mode=1),
@@ -47,6 +50,8 @@ SPMV_VARIANTS = (
     Schedule(group_lanes=16, short_max=256, chunk_nnz=512, unroll=4),
     Schedule(group_lanes=32, short_max=512, chunk_nnz=1024, unroll=4),
     Schedule(group_lanes=64, short_max=4096, chunk_nnz=4096, unroll=8),
+    Schedule(tile_nnz=2048, tile_mode=1, x_window=-1),
+    Schedule(tile_nnz=4096, tile_mode=1, x_window=40960),
 )
 # The plain default-tile SpMV the learned selector is compared with: a whole
 # wavefront per row over fixed blocks of entries -- the structure of the
@@ -194,15 +199,16 @@ class TimingDB:
     def save(self, path: str):
         ids, fam, feats, times, sizes = self.arrays()
         np.savez_compressed(path, op=self.op, device_name=self.device_name,
-                            version=VARIANTS_VERSION, ids=ids, families=fam, features=feats,
-                            times=times, sizes=sizes)
+                            version=VARIANTS_VERSION[self.op], ids=ids, families=fam,
+                            features=feats, times=times, sizes=sizes)
 
     @classmethod
     def load(cls, path: str) -> "TimingDB":
         z = np.load(path, allow_pickle=False)
         db = cls(str(z["op"]), int(z["times"].shape[1]), str(z["device_name"]))
-        if int(z["version"]) != VARIANTS_VERSION:
-            raise ValueError(f"{path}: variant list version {int(z['version'])} != {VARIANTS_VERSION}")
+        want = VARIANTS_VERSION[db.op]
+        if int(z["version"]) != want:
+            raise ValueError(f"{path}: variant list version {int(z['version'])} != {want}")
         for i, f, ft, t, sz in zip(z["ids"], z["families"], z["features"], z["times"], z["sizes"]):
             db.add(int(i), str(f), ft, t, int(sz[0]), int(sz[1]))
         return db

@@ -155,6 +155,26 @@ struct Comp {
   __device__ __forceinline__ float value() const { return s + c; }
 };
 
+// f32c: a compensated fp32 sum held as a value pair (running sum s and the
+// accumulated rounding errors c), the sum type of the tile roles when the
+// accumulator is Comp: adding two of them is a TwoSum of the sums plus the
+// corrections, so a row summed from pieces keeps fp32 arithmetic throughout.
+struct f32c {
+  float s, c;
+};
+template <> __device__ __forceinline__ f32c zero<f32c>() { return {0.f, 0.f}; }
+__device__ __forceinline__ f32c add(f32c a, f32c b) {
+  const float t = a.s + b.s;
+  const float bb = t - a.s;
+  const float err = (a.s - (t - bb)) + (b.s - bb);
+  return {t, a.c + b.c + err};
+}
+// scaling by a power of two (the fp16 row scales) is exact on both parts
+__device__ __forceinline__ f32c scale(f32c a, double s) { return {a.s * (float)s, a.c * (float)s}; }
+// the value a finished sum stands for (other sum types are their own value)
+__device__ __forceinline__ float final_value(f32c a) { return a.s + a.c; }
+template <typename T> __device__ __forceinline__ T final_value(T a) { return a; }
+
 // -------------------------------------------------------- lane exchanges
 //
 // xmov<K>(v): value held by the partner lane in the K-step of a butterfly-like
@@ -193,6 +213,7 @@ template <int K, typename T> __device__ __forceinline__ Plain<T> xmov(const Plai
   o.s = xmov<K>(a.s);
   return o;
 }
+template <int K> __device__ __forceinline__ f32c xmov(f32c a) { return {xmov<K>(a.s), xmov<K>(a.c)}; }
 template <int K> __device__ __forceinline__ Comp xmov(const Comp& a) {
   Comp o;
   o.s = xmov<K>(a.s);

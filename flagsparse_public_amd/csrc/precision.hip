@@ -104,7 +104,7 @@ int launch_split_xg(const fs_csr* M, const VS& vs, const void* xg, void* y, doub
   const int64_t grid = spmv_fill_schedule(p, M, pl, 1);
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr_split: grid of %lld blocks too large", (long long)grid);
-  return spmv_launch_entry<Plain<float>, false, true>(p, grid, st, "fs_spmv_csr_split");
+  return spmv_launch_entry<Comp, false, true>(p, grid, st, "fs_spmv_csr_split");
 }
 
 template <typename VS, typename P, int S>
@@ -124,17 +124,17 @@ int launch_split_s(const fs_csr* M, const VS& vs, const void* x, void* y, double
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr_split: grid of %lld blocks too large", (long long)grid);
   if (pl->tile_nnz > 0 && pl->tile_mode == FS_TILE_ENTRIES)
-    return spmv_launch_entry<Plain<float>, false>(p, grid, st, "fs_spmv_csr_split");
+    return spmv_launch_entry<Comp, false>(p, grid, st, "fs_spmv_csr_split");
   if (pl->tile_nnz > 0) {
     if (tile_lds_bytes<double>(pl->tile_nnz) > 65536)
       return fail(FS_ERR_VALUE, "fs_spmv_csr_split: tile_nnz %lld needs more than 64 KB of LDS",
                   (long long)pl->tile_nnz);
-    spmv_launch_tiles<Plain<float>, 4>(p, grid, st);
+    spmv_launch_tiles<Comp, 4>(p, grid, st);
   } else {
-    spmv_csr_kernel<Plain<float>, S, 4><<<dim3((unsigned)grid), dim3(kBlock), 0, st>>>(p);
+    spmv_csr_kernel<Comp, S, 4><<<dim3((unsigned)grid), dim3(kBlock), 0, st>>>(p);
   }
   const int rc = check_launch("fs_spmv_csr_split");
-  return rc ? rc : spmv_launch_fixup<Plain<float>>(p, pl, st, "fs_spmv_csr_split");
+  return rc ? rc : spmv_launch_fixup<Comp>(p, pl, st, "fs_spmv_csr_split");
 }
 
 template <typename VS, typename P>

@@ -7,8 +7,8 @@
 // block groups pinned to XCDs so each XCD's L2 holds only its own x columns)
 // writes one partial z[i] per compact row.  This pass then forms
 //   y[r] = alpha * sum_k z[compact row (k, r)] + beta * y[r]
-// with the sum taken in slice order in fp64 (complex: two fp64), so the
-// result does not depend on scheduling.
+// with the sum taken in slice order in fp64 (complex: two fp64; with
+// fp32_sums, compensated fp32), so the result does not depend on scheduling.
 //
 // No counterpart in the reference (its SpMV reads x in place,
 // spmv_csr.py:158-184); the row partition and the error contract are those of
@@ -26,13 +26,12 @@ constexpr int kCombineRows = 256;  // rows per block (one per thread)
 // then added to the LDS row sums one slice at a time.
 constexpr int kGroup = 8;
 
-template <typename Z, typename V>
+template <typename Z, typename V, typename S>
 __global__ __launch_bounds__(kCombineRows) void sliced_combine_kernel(
     const Z* __restrict__ z, const int32_t* __restrict__ compact_row,
     const int64_t* __restrict__ bounds, int32_t n_slices, int64_t n_rows, int64_t n_blocks,
     V* __restrict__ y, double alpha, double beta) {
   using C = typename Native<Z>::T;
-  using S = typename TileSum<C>::T;
   __shared__ S acc[kCombineRows];
   const int64_t b = blockIdx.x;
   const int tid = threadIdx.x;
@@ -60,12 +59,12 @@ __global__ __launch_bounds__(kCombineRows) void sliced_combine_kernel(
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
       __syncthreads();  // slice order: rows are distinct inside a slice
-      if (rv[u] >= 0) acc[rv[u]] = add(acc[rv[u]], widen(zv[u]));
+      if (rv[u] >= 0) acc[rv[u]] = add(acc[rv[u]], to_sum<S>(zv[u]));
     }
   }
   __syncthreads();
   const int64_t row = row0 + tid;
-  if (row < n_rows) write_out(y, row, acc[tid], alpha, beta);
+  if (row < n_rows) write_out(y, row, final_value(acc[tid]), alpha, beta);
 }
 
 // The same sums from the row masks: thread t owns row 256 b + t and, slice
@@ -80,13 +79,12 @@ constexpr int kMaskWords = kCombineRows / 32;
 // kRB consecutive 256-row blocks per workgroup (thread t owns row t of each):
 // the blocks' mask words and bounds are staged together and each thread has
 // kRB * kGroup partial loads in flight.
-template <typename Z, typename V, int kRB>
+template <typename Z, typename V, typename S, int kRB>
 __global__ __launch_bounds__(kCombineRows) void sliced_combine_mask_kernel(
     const Z* __restrict__ z, const uint32_t* __restrict__ row_mask,
     const int64_t* __restrict__ bounds, int32_t n_slices, int64_t n_rows, int64_t n_blocks,
     int64_t n_compact, V* __restrict__ y, double alpha, double beta) {
   using C = typename Native<Z>::T;
-  using S = typename TileSum<C>::T;
   static_assert(kRB * kGroup * kMaskWords <= 128 && kRB * kGroup <= 128, "staging threads");
   __shared__ uint32_t words[kRB][kGroup][kMaskWords];
   __shared__ int32_t before[kRB][kGroup][kMaskWords];  // set bits in the words before j
@@ -100,7 +98,7 @@ __global__ __launch_bounds__(kCombineRows) void sliced_combine_mask_kernel(
 #pragma unroll
     for (int r = 0; r < kRB; ++r) {
       const int64_t row = (b0 + r) * kCombineRows + tid;
-      if (row < n_rows) write_out(y, row, acc[r], alpha, beta);
+      if (row < n_rows) write_out(y, row, final_value(acc[r]), alpha, beta);
     }
     return;
   }
@@ -153,16 +151,18 @@ __global__ __launch_bounds__(kCombineRows) void sliced_combine_mask_kernel(
     for (int r = 0; r < kRB; ++r)
 #pragma unroll
       for (int u = 0; u < kGroup; ++u)
-        if ((present[r] >> u) & 1u) acc[r] = add(acc[r], widen(zv[r][u]));
+        if ((present[r] >> u) & 1u) acc[r] = add(acc[r], to_sum<S>(zv[r][u]));
   }
 #pragma unroll
   for (int r = 0; r < kRB; ++r) {
     const int64_t row = (b0 + r) * kCombineRows + tid;
-    if (row < n_rows) write_out(y, row, acc[r], alpha, beta);
+    if (row < n_rows) write_out(y, row, final_value(acc[r]), alpha, beta);
   }
 }
 
-template <typename Z, typename V>
+// S: the sum type -- fp64 (two fp64 for complex) by default, a compensated
+// fp32 pair for fp32 partials with fp32_sums set.
+template <typename Z, typename V, typename S = typename TileSum<typename Native<Z>::T>::T>
 int launch_combine(const fs_slices* sl, const void* z, void* y, double alpha, double beta,
                    hipStream_t st) {
   const int64_t nb = (sl->n_rows + kCombineRows - 1) / kCombineRows;
@@ -170,12 +170,12 @@ int launch_combine(const fs_slices* sl, const void* z, void* y, double alpha, do
   if (sl->row_mask) {
     // two row blocks per workgroup where the partials are at most 8 bytes
     constexpr int kRB = sizeof(typename Native<Z>::T) <= 8 ? 2 : 1;
-    sliced_combine_mask_kernel<Z, V, kRB><<<dim3((unsigned)((nb + kRB - 1) / kRB)),
+    sliced_combine_mask_kernel<Z, V, S, kRB><<<dim3((unsigned)((nb + kRB - 1) / kRB)),
                                             dim3(kCombineRows), 0, st>>>(
         static_cast<const Z*>(z), sl->row_mask, sl->bounds, sl->n_slices, sl->n_rows, nb,
         sl->n_compact, static_cast<V*>(y), alpha, beta);
   } else {
-    sliced_combine_kernel<Z, V><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
+    sliced_combine_kernel<Z, V, S><<<dim3((unsigned)nb), dim3(kCombineRows), 0, st>>>(
         static_cast<const Z*>(z), sl->compact_row, sl->bounds, sl->n_slices, sl->n_rows, nb,
         static_cast<V*>(y), alpha, beta);
   }
@@ -198,8 +198,11 @@ extern "C" int fs_sliced_combine(const fs_slices* sl, const void* z, void* y, do
     return fail(FS_ERR_VALUE, "fs_sliced_combine: null pointer");
   hipStream_t st = as_stream(stream);
   const bool sums = sl->z_type != sl->value_type;
+  if (sl->fp32_sums && (sl->value_type != FS_F32 || sums))
+    return fail(FS_ERR_VALUE, "fs_sliced_combine: fp32_sums is for fp32 partials of fp32 values");
   switch (sl->value_type) {
     case FS_F32:
+      if (!sums && sl->fp32_sums) return launch_combine<float, float, f32c>(sl, z, y, alpha, beta, st);
       if (!sums) return launch_combine<float, float>(sl, z, y, alpha, beta, st);
       if (sl->z_type == FS_F64) return launch_combine<double, float>(sl, z, y, alpha, beta, st);
       break;

@@ -260,6 +260,16 @@ __device__ __forceinline__ double widen(double v) { return v; }
 template <typename T> __device__ __forceinline__ cplx<double> widen(cplx<T> v) {
   return {(double)v.re, (double)v.im};
 }
+
+// Row-sum type of an accumulator's tile roles: fp64 (two fp64 for complex)
+// for Plain accumulators, a compensated fp32 pair for Comp (the split-
+// precision modes, which accumulate in fp32 only).
+template <typename A> struct TileSumOf { using T = typename TileSum<typename A::compute_t>::T; };
+template <> struct TileSumOf<Comp> { using T = f32c; };
+
+// A product or partial as a row-sum term.
+template <typename S, typename C> __device__ __forceinline__ S to_sum(C v) { return widen(v); }
+template <> __device__ __forceinline__ f32c to_sum<f32c, float>(float v) { return {v, 0.f}; }
 __device__ __forceinline__ double mulc(double a, double b) { return a * b; }
 template <typename T> __device__ __forceinline__ cplx<T> mulc(cplx<T> a, cplx<T> b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
@@ -275,7 +285,7 @@ template <typename A, int U, typename VS, typename V, typename I, typename P>
 __device__ __forceinline__ void spmv_tile_role(const SpmvParams<VS, V, I, P>& p, int64_t t,
                                                char* smem) {
   using C = typename A::compute_t;
-  using S = typename TileSum<C>::T;
+  using S = typename TileSumOf<A>::T;
   int64_t* rp = reinterpret_cast<int64_t*>(smem);
   S* prod = reinterpret_cast<S*>(smem + sizeof(int64_t) * (kTileRows + 2));
   const int64_t r0 = p.tile_start[t];
@@ -329,7 +339,7 @@ __device__ __forceinline__ void spmv_tile_role(const SpmvParams<VS, V, I, P>& p,
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int64_t j = base + (int64_t)u * kBlock;
-        if (j < e1) prod[j - e0] = widen(mulc(p.vals.template conv<C>(ar[u]), (C)xr[u]));
+        if (j < e1) prod[j - e0] = to_sum<S>(mulc(p.vals.template conv<C>(ar[u]), (C)xr[u]));
       }
       continue;
     }
@@ -350,7 +360,7 @@ __device__ __forceinline__ void spmv_tile_role(const SpmvParams<VS, V, I, P>& p,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t j = base + (int64_t)u * kBlock;
-      if (j < e1) prod[j - e0] = widen(mulc(a[u], xv[u]));
+      if (j < e1) prod[j - e0] = to_sum<S>(mulc(a[u], xv[u]));
     }
   }
   __syncthreads();
@@ -359,7 +369,7 @@ __device__ __forceinline__ void spmv_tile_role(const SpmvParams<VS, V, I, P>& p,
     if (e - s > kThreadRowMax) continue;
     S acc = zero<S>();
     for (int64_t j = s; j < e; ++j) acc = add(acc, prod[j]);
-    write_row(p.vals, p.y, r0 + i, acc, p.alpha, p.beta);
+    write_row(p.vals, p.y, r0 + i, final_value(acc), p.alpha, p.beta);
   }
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   for (int i = w; i < nr; i += kWavesPerBlock) {
@@ -369,7 +379,7 @@ __device__ __forceinline__ void spmv_tile_role(const SpmvParams<VS, V, I, P>& p,
     acc.init();
     for (int64_t j = s + lane; j < e; j += kWave) acc.add_value(prod[j]);
     group_reduce<kWave>(acc);
-    if (lane == 0) write_row(p.vals, p.y, r0 + i, acc.value(), p.alpha, p.beta);
+    if (lane == 0) write_row(p.vals, p.y, r0 + i, final_value(acc.value()), p.alpha, p.beta);
   }
 }
 
@@ -443,7 +453,7 @@ int64_t spmv_fill_schedule(SpmvParams<VS, V, I, P>& p, const fs_csr* M, const fs
 // Launches the tile-role kernel (chunk + tile blocks) with its dynamic LDS.
 template <typename A, int U, typename VS, typename V, typename I, typename P>
 void spmv_launch_tiles(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st) {
-  using S = typename TileSum<typename A::compute_t>::T;
+  using S = typename TileSumOf<A>::T;
   spmv_tile_kernel<A, U><<<dim3((unsigned)grid), dim3(kBlock), tile_lds_bytes<S>(p.tile_nnz), st>>>(p);
 }
 
@@ -506,7 +516,7 @@ __device__ __forceinline__ void entry_write(const SpmvParams<VS, V, I, P>& p, in
     if constexpr (VS::kRowScale) acc = scale(acc, (double)p.vals.row_scale(row));
     static_cast<S*>(p.row_sums)[row] = acc;
   } else {
-    write_row(p.vals, p.y, row, acc, p.alpha, p.beta);
+    write_row(p.vals, p.y, row, final_value(acc), p.alpha, p.beta);
   }
 }
 
@@ -723,14 +733,14 @@ __device__ __forceinline__ void entry_rows(const SpmvParams<VS, V, I, P>& p,
     if (w == 0) {
       Plain<S> acc;
       acc.init();
-      for (int j = lane; j < head; j += kWave) acc.add_value(widen(prod[j]));
+      for (int j = lane; j < head; j += kWave) acc.add_value(to_sum<S>(prod[j]));
       group_reduce<kWave>(acc);
       if (lane == 0) carries[2 * t] = acc.value();
     }
   } else if (head > 0) {
     Plain<S> acc;
     acc.init();
-    for (int j = tid; j < head; j += kBlock) acc.add_value(widen(prod[j]));
+    for (int j = tid; j < head; j += kBlock) acc.add_value(to_sum<S>(prod[j]));
     group_reduce<kWave>(acc);
     if (lane == 0) red[w] = acc.value();
     __syncthreads();
@@ -763,7 +773,7 @@ __device__ __forceinline__ void entry_rows(const SpmvParams<VS, V, I, P>& p,
         continue;
       }
       S acc = zero<S>();
-      for (int j = s; j < e; ++j) acc = add(acc, widen(prod[j]));
+      for (int j = s; j < e; ++j) acc = add(acc, to_sum<S>(prod[j]));
       const int64_t row = rb + i;
       entry_finish<A>(p, row, acc, straddle && row == r1 - 1, carries, t);
     }
@@ -775,7 +785,7 @@ __device__ __forceinline__ void entry_rows(const SpmvParams<VS, V, I, P>& p,
       const int s = bnd[i], e = bnd[i + 1];
       Plain<S> acc;
       acc.init();
-      for (int j = s + lane; j < e; j += kWave) acc.add_value(widen(prod[j]));
+      for (int j = s + lane; j < e; j += kWave) acc.add_value(to_sum<S>(prod[j]));
       group_reduce<kWave>(acc);
       const int64_t row = rb + i;
       if (lane == 0) entry_finish<A>(p, row, acc.value(), straddle && row == r1 - 1, carries, t);
@@ -800,7 +810,7 @@ template <typename C, typename S, int T> struct EntryLds {
 template <typename A, int E, bool VEC, bool XG, typename VS, typename V, typename I, typename P>
 __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I, P> p) {
   using C = typename A::compute_t;
-  using S = typename TileSum<C>::T;
+  using S = typename TileSumOf<A>::T;
   extern __shared__ __align__(16) char smem[];
   const EntryLds<C, S, E * kBlock> lds(smem);
   S* carries = reinterpret_cast<S*>(p.partials);
@@ -832,7 +842,7 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I,
 // tile that owns the next row (or the last tile).
 template <typename A, typename VS, typename V, typename I, typename P>
 __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS, V, I, P> p) {
-  using S = typename TileSum<typename A::compute_t>::T;
+  using S = typename TileSumOf<A>::T;
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t + 1 >= p.n_tiles) return;  // the last tile ends at nnz: no row runs past it
   const int64_t r0 = p.tile_start[t], r1 = p.tile_start[t + 1];
@@ -840,9 +850,30 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS,
   if ((int64_t)p.ptr[r1] <= p.tile_entry[t + 1]) return;
   const S* carries = reinterpret_cast<const S*>(p.partials);
   S acc = carries[2 * t + 1];
-  for (int64_t u = t + 1; u < p.n_tiles; ++u) {
-    acc = add(acc, carries[2 * u]);
-    if (p.tile_start[u + 1] > p.tile_start[u]) break;
+  // the following tiles kWalk at a time: their heads and row starts are
+  // loaded together (indices clamped to the last tile), then added in tile
+  // order up to the tile that owns the next row -- the same sum as one tile
+  // at a time, without a dependent load per tile
+  constexpr int kWalk = 8;
+  const int64_t last = p.n_tiles - 1;
+  bool done = false;
+  for (int64_t u0 = t + 1; u0 < p.n_tiles && !done; u0 += kWalk) {
+    S head[kWalk];
+    int64_t ts[kWalk + 1];
+#pragma unroll
+    for (int k = 0; k < kWalk; ++k) {
+      const int64_t u = u0 + k < last ? u0 + k : last;
+      head[k] = carries[2 * u];
+      ts[k] = p.tile_start[u];
+    }
+    ts[kWalk] = p.tile_start[u0 + kWalk < p.n_tiles ? u0 + kWalk : p.n_tiles];
+#pragma unroll
+    for (int k = 0; k < kWalk; ++k) {
+      if (done || u0 + k >= p.n_tiles) break;
+      acc = add(acc, head[k]);
+      const int64_t next = k + 1 < kWalk ? ts[k + 1] : ts[kWalk];
+      if (u0 + k == last || next > ts[k]) done = true;
+    }
   }
   entry_write(p, r1 - 1, acc);
 }

@@ -10,13 +10,15 @@
                          the row's max |v| (11 bits; 2 bytes + 4 per row).
 
 ``flagsparse_spmv_csr_split`` then computes y = alpha A x + beta y with fp32
-x / y and fp32 products.  Row sums: fp64 in the tile roles (the default entry
-tiles), compensated TwoSum fp32 summation in the chunked long-row role of the
-row-group schedules.  When x is larger than 4 MB the default schedule also
-uses the column-sliced layout (slicing.py) with the planes permuted into
-slice-major order.  The
-reference stores values only in the call's dtype; these modes have no
-counterpart there and are measured beside the plain fp32 kernel.
+x / y and fp32 products, and accumulates in fp32 only: every row sum is a
+compensated (TwoSum) fp32 sum -- lane accumulators, the tile roles' row sums
+and the carries of rows cut by tile boundaries, chunk partials of long rows,
+and the column-sliced combine -- so no fp64 arithmetic is used (fp64 is kept
+for float64 calls and the oracle check).  When x is larger than 8 MB the
+default schedule also uses the column-sliced layout (slicing.py) with the
+planes permuted into slice-major order.  The reference stores values only in
+the call's dtype; these modes have no counterpart there and are measured
+beside the plain fp32 kernel.
 """
 
 from __future__ import annotations
@@ -111,7 +113,8 @@ def _sliced_split(prepared, sch):
     if sl is None:
         sl = SlicedCsr(None, prepared.kernel_indices, prepared.kernel_indptr, prepared.n_rows,
                        prepared.n_cols, sch.x_slice_cols, sch, _lib.FS_ACC_NATIVE,
-                       planes=(prepared.hi, prepared.lo), value_dtype=torch.float32)
+                       planes=(prepared.hi, prepared.lo), value_dtype=torch.float32,
+                       fp32_sums=True)
         sl.row_scale = (prepared.row_scale[sl.compact_row.to(torch.int64)].contiguous()
                         if prepared.row_scale is not None else None)
         prepared._plans[key] = sl

@@ -36,6 +36,7 @@ class Schedule:
     tile_mode: int = 0        # SpMV with tile_nnz > 0: 0 row tiles, 1 entry tiles
     x_window: int = 0         # SpMV fp32 / fp64 with entry tiles: != 0 gathers x from
                               # windows staged in LDS first (columns per window; -1: 128 KB)
+    b_panels: int = 1         # SpMM: launches over column panels of B (1: all columns)
 
     def key(self) -> "Schedule":
         """Cache key (a frozen dataclass hashes and compares by its fields)."""

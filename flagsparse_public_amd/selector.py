@@ -179,7 +179,7 @@ class Selector:
     def to_json(self) -> dict:
         return {
             "op": self.op,
-            "variants_version": VARIANTS_VERSION,
+            "variants_version": VARIANTS_VERSION[self.op],
             "in_dim": self.mlp.in_dim, "hidden": self.mlp.hidden, "out_dim": self.mlp.out_dim,
             "params": self.mlp.params.cpu().numpy().astype(float).tolist(),
             "mu": self.mu.cpu().numpy().astype(float).tolist(),
@@ -195,7 +195,7 @@ class Selector:
     def load(cls, path: str, device) -> "Selector":
         with open(path) as fh:
             d = json.load(fh)
-        if int(d["variants_version"]) != VARIANTS_VERSION:
+        if int(d["variants_version"]) != VARIANTS_VERSION.get(d["op"], -1):
             raise ValueError(f"{path}: variant list version mismatch")
         mlp = Mlp(int(d["out_dim"]), device, hidden=int(d["hidden"]), in_dim=int(d["in_dim"]),
                   params=np.asarray(d["params"], np.float32))

@@ -59,13 +59,15 @@ class SlicedCsr:
     """Slice-major copy of one CSR matrix plus its compact-row plan."""
 
     def __init__(self, data, indices, indptr, n_rows, n_cols, slice_cols, schedule,
-                 accumulate, planes=(), value_dtype=None, sums=False):
+                 accumulate, planes=(), value_dtype=None, sums=False, fp32_sums=False):
         """``data`` may be None when the values live in other per-entry
         ``planes`` (split-precision storage, value_dtype float32); every plane
         is permuted into slice-major order with the entries.  ``sums``: the
         partials are the kernels' raw fp64 row sums (complex: two fp64)
         instead of values rounded to the value dtype -- the default fp32 path,
-        which accumulates in fp64 and rounds once."""
+        which accumulates in fp64 and rounds once.  ``fp32_sums``: the combine
+        adds fp32 partials in compensated fp32 instead of fp64 (the
+        split-precision modes, which use no fp64)."""
         value_dtype = data.dtype if data is not None else value_dtype
         if value_dtype not in SLICED_DTYPES:
             raise TypeError(f"column slicing supports {SLICED_DTYPES}, got {value_dtype}")
@@ -141,7 +143,8 @@ class SlicedCsr:
                                   compact_row=_lib.ptr(self.compact_row),
                                   bounds=_lib.ptr(self.bounds), rows_per_block=COMBINE_ROWS,
                                   z_type=_lib.value_code(z_dtype),
-                                  row_mask=_lib.ptr(self.row_mask))
+                                  row_mask=_lib.ptr(self.row_mask),
+                                  fp32_sums=int(bool(fp32_sums)), reserved=0)
 
     def run(self, x, y, alpha, beta, device):
         """y = alpha A x + beta y: compact-row partials, then the combine."""

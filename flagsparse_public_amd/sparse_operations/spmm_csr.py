@@ -239,12 +239,24 @@ def _spmm_target(out, beta):
 
 
 def _launch_spmm(csr, plan, B, n_rows, n_dense, dtype, device, out=None, alpha=1.0, beta=0.0,
-                 col_tile=0):
+                 col_tile=0, panels=1):
+    """One launch over all of B's columns, or with panels > 1 one launch per
+    column panel (panel width a multiple of 4), panel after panel, so the B
+    columns gathered at a time are a 1/panels share of B."""
     C = out if out is not None else torch.empty((n_rows, n_dense), dtype=dtype, device=device)
     if n_rows == 0 or n_dense == 0:
         return C
-    _lib.spmm(csr, B, B.stride(0) if B.numel() else n_dense, C, C.stride(0), n_dense, alpha, beta,
-              plan.struct(n_dense), col_tile, device)
+    ldb = B.stride(0) if B.numel() else n_dense
+    width = n_dense
+    if panels > 1:
+        per = -(-n_dense // panels)
+        width = -(-per // 4) * 4
+    for c0 in range(0, n_dense, width):
+        w = min(width, n_dense - c0)
+        Bp = B[:, c0:c0 + w] if c0 or w != n_dense else B
+        Cp = C[:, c0:c0 + w] if c0 or w != n_dense else C
+        _lib.spmm(csr, Bp, ldb, Cp, C.stride(0), w, alpha, beta, plan.struct(w),
+                  min(col_tile, w) if col_tile > 0 else 0, device)
     return C
 
 
@@ -509,7 +521,7 @@ def _hip_spmm_csr_impl_opt_prepared(prepared, B, out=None, alpha=1.0, beta=0.0, 
     sch, col_tile = spmm_launch_schedule(prepared, n_dense, schedule)
     plan = prepared.plan(_lib.FS_ACC_NATIVE, n_dense, sch)
     C = _launch_spmm(prepared.csr_struct(), plan, B, prepared.n_rows, n_dense, B.dtype, B.device,
-                     out, alpha, beta, col_tile)
+                     out, alpha, beta, col_tile, sch.b_panels)
     return C, False
 
 

@@ -95,7 +95,7 @@ class PreparedCsrSpmv:
         "transpose", "op", "index_fallback_policy", "index_fallback_applied",
         "index_fallback_reason", "_baseline_compute_dtype", "_baseline_data",
         "schedule", "_plans", "_csr", "row_lengths", "schedule_record",
-        "_bucket_max_segments",
+        "_bucket_max_segments", "_stats",
     )
 
     def __init__(self, data, kernel_indices, kernel_indptr, shape, n_rows, n_cols, block_nnz,
@@ -132,6 +132,7 @@ class PreparedCsrSpmv:
         self.row_lengths = row_lengths
         self._plans = {}
         self._csr = None
+        self._stats = None  # validation statistics of the caller's matrix (prepare_spmv_csr)
 
     @property
     def opt_buckets(self):
@@ -292,7 +293,7 @@ def _prepare_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy
     stats_dev = _lib.analyze_device(_lib.make_csr(data, indices, indptr, n_rows, n_cols), data.device)
     stats = stats_dev.cpu().tolist()
     if _stats_out is not None:
-        _stats_out.append(stats_dev)
+        _stats_out.append((stats_dev, stats))
     if stats[0] != 0:
         raise ValueError("indptr must start at zero")
     if stats[1] != nnz:
@@ -338,9 +339,11 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
     if op is not None and bool(transpose) and op_code == SPMV_OP_NON:
         raise ValueError("transpose=True conflicts with op=non")
     transpose = _spmv_op_transposes(op_code)
+    caller_stats = []
     if transpose:
         data, indices, indptr, *_ = _prepare_spmv_csr_matrix(
-            data, indices, indptr, shape, index_fallback_policy=index_fallback_policy)
+            data, indices, indptr, shape, index_fallback_policy=index_fallback_policy,
+            _stats_out=caller_stats)
         if op_code == SPMV_OP_CONJ_TRANS and _is_complex_dtype(data.dtype):
             data = data.conj().resolve_conj()
         data, indices, indptr, shape = _transpose_csr_for_spmv(data, indices, indptr, shape)
@@ -357,7 +360,7 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
             max_segments_use = max((max_row_nnz + block_nnz_use - 1) // block_nnz_use, 1)
     else:
         max_segments_use = max_segments
-    sch, how = choose_schedule("spmv", schedule, data, n_rows, n_cols, stats_out[0])
+    sch, how = choose_schedule("spmv", schedule, data, n_rows, n_cols, stats_out[0][0])
     prepared = PreparedCsrSpmv(
         data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
         n_rows=n_rows, n_cols=n_cols, block_nnz=block_nnz_use, max_segments=max_segments_use,
@@ -368,6 +371,78 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
         sch = exhaustive_spmv_schedule(prepared)
         prepared.schedule = sch
     prepared.schedule_record = _schedule_record(sch, how)
+    prepared._stats = (caller_stats or stats_out)[0][1]
+    return prepared
+
+
+class _UnpreparedCache:
+    """Prepared objects of recent unprepared calls (``flagsparse_spmv_csr``
+    given data / indices / indptr and no ``prepared``), so a repeated call
+    on the same matrix does not rebuild its schedule, layouts and selection.
+
+    An entry is reused only for the very same tensor objects (held by weak
+    reference) with unchanged version counters (every in-place write through
+    torch bumps them) and data pointers, the same shape and options.  Every
+    reuse still runs the validation pass on the caller's tensors (one
+    fs_csr_analyze and its host read) and raises exactly as preparing again
+    would; if that pass's structure statistics (row-length histogram, column
+    range, ...) differ from the cached ones, the matrix is prepared again.
+    """
+
+    def __init__(self, size=4):
+        self.size = size
+        self._entries = []  # (key, weakrefs, prepared), most recent last
+
+    @staticmethod
+    def _key(tensors, shape, options):
+        return (tuple((t.data_ptr(), t._version, t.dtype, tuple(t.shape), t.stride())
+                      for t in tensors), (int(shape[0]), int(shape[1])), options)
+
+    def lookup(self, tensors, shape, options):
+        key = self._key(tensors, shape, options)
+        for i, (k, refs, prepared) in enumerate(self._entries):
+            if k == key and all(r() is t for r, t in zip(refs, tensors)):
+                self._entries.append(self._entries.pop(i))
+                return prepared
+        return None
+
+    def store(self, tensors, shape, options, prepared):
+        import weakref
+        key = self._key(tensors, shape, options)
+        # drop entries of the same key and stale ones for the same tensor
+        # objects (their contents have changed since)
+        self._entries = [e for e in self._entries
+                         if e[0] != key and not (e[0][1:] == key[1:]
+                                                 and all(r() is t for r, t in zip(e[1], tensors)))]
+        self._entries.append((key, [weakref.ref(t) for t in tensors], prepared))
+        del self._entries[:-self.size]
+
+    def clear(self):
+        self._entries = []
+
+
+_UNPREPARED = _UnpreparedCache()
+
+
+def _prepare_for_call(data, indices, indptr, shape, block_nnz, max_segments, op_code,
+                      index_fallback_policy):
+    """The prepared object of an unprepared call: a cached one after a fresh
+    validation pass (see _UnpreparedCache), else prepare_spmv_csr."""
+    tensors = (data, indices, indptr)
+    options = (block_nnz, max_segments, op_code, index_fallback_policy)
+    if all(torch.is_tensor(t) for t in tensors):
+        prepared = _UNPREPARED.lookup(tensors, shape, options)
+        if prepared is not None:
+            stats = []
+            _prepare_spmv_csr_matrix(data, indices, indptr, shape,
+                                     index_fallback_policy=index_fallback_policy,
+                                     _stats_out=stats)
+            if stats[0][1] == prepared._stats:
+                return prepared
+    prepared = prepare_spmv_csr(data, indices, indptr, shape, block_nnz=block_nnz,
+                                max_segments=max_segments, op=op_code,
+                                index_fallback_policy=index_fallback_policy)
+    _UNPREPARED.store(tensors, shape, options, prepared)
     return prepared
 
 
@@ -384,7 +459,7 @@ def _launch_spmv(prepared, x, accumulate, out=None, alpha=1.0, beta=0.0, schedul
         return y
     sch = schedule or prepared.schedule
     if (sch is not None and sch.x_window != 0 and prepared.data.dtype in XWIN_DTYPES
-            and prepared.data.numel() > 0):
+            and 0 < prepared.data.numel() <= 2**31 - 1):
         # two streaming passes: x gathered from LDS windows, then entry tiles
         key = ("xwin", sch.key())
         xw = prepared._plans.get(key)
@@ -525,9 +600,8 @@ def flagsparse_spmv_csr(data=None, indices=None, indptr=None, x=None, shape=None
         if any(a is None for a in (data, indices, indptr, shape)):
             raise ValueError(
                 "data, indices, indptr, and shape are required when prepared is not provided")
-        prepared = prepare_spmv_csr(data, indices, indptr, shape, block_nnz=block_nnz,
-                                    max_segments=max_segments, op=op_code,
-                                    index_fallback_policy=index_fallback_policy)
+        prepared = _prepare_for_call(data, indices, indptr, shape, block_nnz, max_segments,
+                                     op_code, index_fallback_policy)
     else:
         if op_explicit and op_code != prepared.op:
             raise ValueError(f"op={_spmv_op_to_name(op_code)} does not match "

@@ -47,8 +47,9 @@ def _log(msg):
 
 def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=100,
             deadline=None):
-    """Time every variant on each matrix; returns False when the deadline
-    (time.time() value) stopped collection early."""
+    """Time every variant of each operation in ``dbs`` on each matrix;
+    returns False when the deadline (time.time() value) stopped collection
+    early."""
     dense_cache = {}
     for k, seed in enumerate(seeds):
         if deadline is not None and time.time() > deadline:
@@ -64,21 +65,26 @@ def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=10
         g = torch.Generator(device=device)
         g.manual_seed(seed + 1)
         x = torch.rand(n, generator=g, device=device) * 2 - 1
-        tv = tune_spmv(pv, x, reps=reps)
-        dbs["spmv"].add(seed, family, feats, tv, n, nnz)
-        if n not in dense_cache:
-            dense_cache.clear()
-            dense_cache[n] = torch.rand((n, 64), generator=g, device=device) * 2 - 1
-        pm = prepare_spmm_csr_opt(vals, cols, indptr, (n, n), schedule="default")
-        tm = tune_spmm(pm, dense_cache[n], reps=max(3, reps - 2))
-        dbs["spmm64"].add(seed, family, feats, tm, n, nnz)
-        del pv, pm, indptr, cols, vals, x
+        best = []
+        if "spmv" in dbs:
+            tv = tune_spmv(pv, x, reps=reps)
+            dbs["spmv"].add(seed, family, feats, tv, n, nnz)
+            best.append(f"spmv best {tv.min():.1f}us")
+        if "spmm64" in dbs:
+            if n not in dense_cache:
+                dense_cache.clear()
+                dense_cache[n] = torch.rand((n, 64), generator=g, device=device) * 2 - 1
+            pm = prepare_spmm_csr_opt(vals, cols, indptr, (n, n), schedule="default")
+            tm = tune_spmm(pm, dense_cache[n], reps=max(3, reps - 2))
+            dbs["spmm64"].add(seed, family, feats, tm, n, nnz)
+            best.append(f"spmm best {tm.min():.1f}us")
+            del pm
+        del pv, indptr, cols, vals, x
         if save is not None and (k + 1) % save_every == 0:
             save()
         if (k + 1) % 20 == 0 or k + 1 == len(seeds) or n >= 1_000_000:
             _log(f"{label} {k + 1}/{len(seeds)} matrices, {time.time() - t0:.0f}s "
-                 f"(last: {family} n={n} nnz={nnz}, spmv best {tv.min():.1f}us, "
-                 f"spmm best {tm.min():.1f}us)")
+                 f"(last: {family} n={n} nnz={nnz}, {', '.join(best)})")
     return True
 
 
@@ -140,13 +146,20 @@ def main(argv=None):
     ap.add_argument("--collect-only", action="store_true")
     ap.add_argument("--time-budget", type=float, default=0.0,
                     help="stop collecting after this many seconds (saved; resume later)")
+    ap.add_argument("--ops", default="spmv,spmm64",
+                    help="operations to collect and train (the others' files are kept)")
     args = ap.parse_args(argv)
+    ops = [o for o in args.ops.split(",") if o]
+    for o in ops:
+        if o not in ("spmv", "spmm64"):
+            raise SystemExit(f"unknown op {o}")
     os.makedirs(args.out, exist_ok=True)
     device = torch.device("cuda", 0)
     name = torch.cuda.get_device_name(device)
     t0 = time.time()
-    paths = {s: {op: os.path.join(args.out, f"{op}_{s}_db.npz") for op in ("spmv", "spmm64")}
+    paths = {s: {op: os.path.join(args.out, f"{op}_{s}_db.npz") for op in ops}
              for s in ("train", "test")}
+    n_variants = {"spmv": len(SPMV_VARIANTS), "spmm64": len(SPMM_VARIANTS)}
     dbs = {}
     deadline = t0 + args.time_budget if args.time_budget > 0 else None
     complete = True
@@ -158,9 +171,8 @@ def main(argv=None):
         if args.resume and all(os.path.exists(p) for p in paths[split].values()):
             have = {op: TimingDB.load(p) for op, p in paths[split].items()}
         if not have:
-            have = {"spmv": TimingDB("spmv", len(SPMV_VARIANTS), name),
-                    "spmm64": TimingDB("spmm64", len(SPMM_VARIANTS), name)}
-        done = set(have["spmv"].ids)
+            have = {op: TimingDB(op, n_variants[op], name) for op in ops}
+        done = set(have[ops[0]].ids)
         seeds = [s for seed0, count in ranges for s in range(seed0, seed0 + count) if s not in done]
         def save(have=have, split=split):
             for op, db in have.items():
@@ -171,11 +183,18 @@ def main(argv=None):
             save()
         dbs[split] = have
     if args.collect_only or not complete:
-        _log(f"collected: test {len(dbs['test']['spmv'])}, train {len(dbs['train']['spmv'])}")
+        _log(f"collected: test {len(dbs['test'][ops[0]])}, train {len(dbs['train'][ops[0]])}")
         return 0
-    report = {"device": name, "n_train": len(dbs["train"]["spmv"]),
-              "n_test": len(dbs["test"]["spmv"])}
-    for op, default_index in (("spmv", SPMV_DEFAULT_INDEX), ("spmm64", SPMM_DEFAULT_INDEX)):
+    eval_path = os.path.join(args.out, "eval.json")
+    report = {}
+    if os.path.exists(eval_path):  # keep the other operations' reports
+        with open(eval_path) as fh:
+            report = json.load(fh)
+    report.update({"device": name, "n_train": len(dbs["train"][ops[0]]),
+                   "n_test": len(dbs["test"][ops[0]])})
+    defaults = {"spmv": SPMV_DEFAULT_INDEX, "spmm64": SPMM_DEFAULT_INDEX}
+    for op in ops:
+        default_index = defaults[op]
         _, _, Xtr, Ttr, _ = dbs["train"][op].arrays()
         sel = Selector.train(op, Xtr, Ttr, device, epochs=args.epochs,
                              log=lambda m, op=op: _log(f"{op}: {m}"))
@@ -191,7 +210,7 @@ def main(argv=None):
         sel.save(os.path.join(args.out, f"{op}_f32_mlp.json"))
         _log(f"{op}: held-out regret {ev['geomean_regret']:.3f}, "
              f"speed-up vs default {ev['geomean_speedup_vs_default']:.3f}")
-    with open(os.path.join(args.out, "eval.json"), "w") as fh:
+    with open(eval_path, "w") as fh:
         json.dump(report, fh, indent=1)
     print(json.dumps(report), flush=True)
     return 0

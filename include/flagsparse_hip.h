@@ -220,7 +220,7 @@ const char *fs_mtx_last_error(void);
  * writes one partial per compact row into z (as y, or as raw fp64 row sums
  * through fs_plan.row_sums); fs_sliced_combine then forms
  * y[r] = alpha * sum over slices of z[compact row (k, r)] + beta * y[r],
- * summed in slice order in fp64.
+ * summed in slice order in fp64 (or compensated fp32, fs_slices.fp32_sums).
  */
 typedef struct fs_slices {
   int32_t value_type;        /* of y: FS_F32, FS_F64, FS_C64 or FS_C128    */
@@ -242,6 +242,10 @@ typedef struct fs_slices {
                                 partial in slice k is found by counting the
                                 set bits before it, and compact_row is not
                                 read.                                       */
+  int32_t fp32_sums;         /* 1: add fp32 partials of fp32 values in
+                                compensated fp32 (the split-precision
+                                modes); 0: in fp64                          */
+  int32_t reserved;
 } fs_slices;
 
 int fs_sliced_combine(const fs_slices *sl, const void *z, void *y, double alpha, double beta,

@@ -109,3 +109,55 @@ def test_split_rejects_non_fp32():
     with pytest.raises(ValueError):
         prepare_spmv_csr_split(d.float(), torch.zeros(2, dtype=torch.int32, device="cuda"),
                                torch.tensor([0, 2], device="cuda"), (1, 1), value_format="fp8")
+
+
+@pytest.mark.slow
+def test_bf16x2_powerlaw_4m_full_size(orc):
+    """bf16 hi + lo storage on the 4M-row power-law matrix (48.6M entries) with
+    the library's schedule: kernel parity on the decoded values (1e-5 of
+    sum|a x| per row) and the stated bound against the fp32 matrix."""
+    ip, cols, vals = synthetic.spmv_powerlaw_4m()
+    n = ip.size - 1
+    print(f"[bf16x2] powerlaw_4m nnz={vals.size}", flush=True)
+    x = synthetic.dense_uniform(202, n)
+    p = prepare_spmv_csr_split(to_dev(vals), to_dev(cols), to_dev(ip), (n, n), "bf16x2")
+    y = host64(flagsparse_spmv_csr_split(to_dev(x), p))
+    dec = decoded_values(p).cpu().numpy()
+    y_dec = orc.spmv(ip, cols, dec, x)
+    assert row_rel_err(y, y_dec, orc.spmv_abs(ip, cols, dec, x)).max() <= 1e-5
+    y64 = orc.spmv(ip, cols, vals, x)
+    cond = orc.spmv_abs(ip, cols, vals, x)
+    assert row_rel_err(y, y64, cond).max() <= BOUND_VS_FP32["bf16x2"]
+
+
+@pytest.mark.parametrize("fmt", ["bf16x2", "bf16"])
+@pytest.mark.parametrize("sch", [
+    None,                                                    # library schedule
+    Schedule(tile_nnz=2048, tile_mode=1),                    # entry tiles, carries
+    Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=4096),  # sliced, compensated combine
+    Schedule(group_lanes=8, short_max=2048, chunk_nnz=8192, unroll=4, tile_nnz=2048),
+    Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=4),
+])
+def test_split_long_rows_are_compensated(fmt, sch):
+    """A 60,001-entry row 2^13, then 60,000 terms 2^-13 (x = 1), all exact in
+    bf16: each small term is below half an ulp of the running fp32 sum, so
+    plain fp32 accumulation returns 8192 (error 7.3, 9e-4 of sum|a x|).  The
+    split modes sum in compensated fp32: the row comes out exact
+    (8199.32421875), or within one rounding per column slice when sliced."""
+    n_long = 60_001
+    n = 9000
+    lens = np.ones(n, np.int64)
+    lens[7] = n_long
+    ip = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    cols = np.concatenate([np.arange(lens[r]) % n for r in range(n)]).astype(np.int32)
+    vals = np.full(ip[-1], 0.5, np.float32)
+    vals[ip[7]] = 2.0 ** 13
+    vals[ip[7] + 1:ip[8]] = 2.0 ** -13
+    x = np.ones(n, np.float32)
+    p = prepare_spmv_csr_split(to_dev(vals), to_dev(cols), to_dev(ip), (n, n), fmt,
+                               schedule=sch)
+    y = host64(flagsparse_spmv_csr_split(to_dev(x), p))
+    # exact, up to one fp32 rounding of each column-slice partial (8 slices)
+    assert abs(y[7] - (8192.0 + 60_000 * 2.0 ** -13)) <= 8 * 2.0 ** -11
+    assert np.all(y[np.arange(n) != 7] == 0.5)
