@@ -32,7 +32,9 @@ reference (SciPy's CSR product in float64):
     against exhaustive autotuning (every variant timed) and the plain
     default-tile kernel;
   * unprepared_calls: flagsparse_spmv_csr(data, indices, indptr, x, shape)
-    repeated on the same tensors (the reference's default call pattern).
+    repeated on the same tensors (the reference's default call pattern);
+  * modes_stream_bound: the split-precision modes beside plain fp32 on a
+    4M-row banded matrix, where value bytes are a third of the traffic.
 --quick skips them.
 
     python bench.py [--gpus 1] [--steps 200] [--warmup 20] [--path opt|default] [--dtype f32|f64]
@@ -146,6 +148,19 @@ def library_cpu_paths(ip, cols, vals, x, nbytes, reps=5):
         sec = timed(lambda: torch.mv(A, xt))
         out.append({"library": f"torch CPU CSR mv ({vals.dtype})", "ms": round(sec * 1e3, 2),
                     "GB/s": round(nbytes / sec / 1e9, 3), "threads": torch.get_num_threads()})
+        if vals.dtype == np.float32:
+            # the reference's own correctness oracle: torch sparse CSR in the
+            # promoted dtype (fp32 -> fp64, tests/test_spmv.py:302-385)
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore", UserWarning)
+                A64 = torch.sparse_csr_tensor(A.crow_indices(), A.col_indices(),
+                                              A.values().to(torch.float64), size=(n, n))
+            x64 = xt.to(torch.float64)
+            sec = timed(lambda: torch.mv(A64, x64))
+            out.append({"library": "torch CPU CSR mv, promoted to float64 (the reference's "
+                                   "oracle)", "ms": round(sec * 1e3, 2),
+                        "GB/s": round(nbytes / sec / 1e9, 3), "threads": torch.get_num_threads()})
+            del A64
         del A
     except Exception as exc:
         out.append({"library": "torch CPU", "error": str(exc)})
@@ -538,6 +553,46 @@ def sub_selection(d_spmv, spmv_prepared, steps):
     return out
 
 
+def sub_modes_banded(steps):
+    """The split-precision modes on a stream-bound SpMV: a 4M-row band of 13
+    diagonals (52M entries; every x gather hits L2), so the value bytes are a
+    third of the traffic.  Each mode beside the plain fp32 kernel (library
+    schedules), with its error against the fp64 reference."""
+    import numpy as np
+    import torch
+    from flagsparse_public_amd import flagsparse_spmv_csr, prepare_spmv_csr, synthetic
+    from flagsparse_public_amd.precision import (FORMAT_BYTES, flagsparse_spmv_csr_split,
+                                                 prepare_spmv_csr_split)
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    ip, cols, vals = synthetic.banded_csr(4_000_000, 13, 6)
+    n, nnz = ip.size - 1, int(ip[-1])
+    x = synthetic.dense_uniform(14, n)
+    d = [torch.from_numpy(a).to(dev) for a in (vals, cols, ip.astype(np.int32))]
+    dx = torch.from_numpy(x).to(dev)
+    A = _scipy_csr(ip, cols, vals, (n, n))
+    ref, cond = A @ x.astype(np.float64), abs(A) @ np.abs(x.astype(np.float64))
+    p = prepare_spmv_csr(*d, (n, n))
+    us = _event_time(lambda: flagsparse_spmv_csr(x=dx, prepared=p, use_opt=True), steps, stream)
+    y = flagsparse_spmv_csr(x=dx, prepared=p, use_opt=True).cpu().numpy()
+    b = spmv_bytes(n, nnz)
+    out = {"matrix": f"banded, 13 diagonals, n={n}, nnz={nnz}",
+           "fp32": {"us": round(us, 2), "GB/s": round(b / us / 1e3, 1),
+                    "schedule": repr(p.schedule), "max_rel_err_vs_fp64": _rel_err(y, ref, cond)}}
+    for fmt in ("bf16x2", "bf16", "f16_rowscale"):
+        ps = prepare_spmv_csr_split(*d, (n, n), fmt)
+        us = _event_time(lambda: flagsparse_spmv_csr_split(dx, ps), steps, stream)
+        ys = flagsparse_spmv_csr_split(dx, ps).cpu().numpy()
+        per_entry, per_row = FORMAT_BYTES[fmt]
+        bm = (n + 1) * 4 + nnz * (4 + per_entry + 4) + n * (4 + per_row)
+        out[fmt] = {"us": round(us, 2), "GB/s": round(bm / us / 1e3, 1),
+                    "speedup_vs_fp32": round(out["fp32"]["us"] / us, 3),
+                    "max_rel_err_vs_fp64": _rel_err(ys, ref, cond)}
+        del ps
+    _log(f"modes on the banded matrix: {out}")
+    return out
+
+
 def sub_unprepared(d_spmv, prepared_us, reps=20):
     """Unprepared calls flagsparse_spmv_csr(data, indices, indptr, x, shape)
     -- the reference's default call pattern -- on the 4M-row matrix and on
@@ -839,6 +894,8 @@ def main(argv=None):
         torch.cuda.empty_cache()
         work["unprepared_calls"] = sub_unprepared((d_vals, d_cols, d_ip, d_x),
                                                   per_launch_ms * 1e3)
+        torch.cuda.empty_cache()
+        work["modes_stream_bound"] = sub_modes_banded(sub_steps)
         torch.cuda.empty_cache()
         work["spmv_powerlaw_4m_f64"] = sub_spmv_f64(sub_steps)
         torch.cuda.empty_cache()

@@ -18,13 +18,24 @@
 
 namespace fs {
 
+// Value sources of the storage formats.  get4 (entry tiles with 16-byte
+// column / value loads) reads four consecutive 2-byte values of a plane with
+// one 8-byte load; it needs j % 4 == 0 and 8-byte aligned planes.
 struct SplitBf16Vals {
   const uint16_t* hi;
   const uint16_t* lo;
   template <typename C> __device__ __forceinline__ C get(int64_t j) const {
     return (C)(bf16_to_f32(hi[j]) + bf16_to_f32(lo[j]));
   }
-  static constexpr bool kVec = false;
+  static constexpr bool kVec = true;
+  template <typename C> __device__ __forceinline__ void get4(int64_t j, C (&o)[4]) const {
+    uint16_t h[4], l[4];
+    load4(hi, j, h);
+    load4(lo, j, l);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = (C)(bf16_to_f32(h[k]) + bf16_to_f32(l[k]));
+  }
+  bool vec_aligned() const { return ((uintptr_t)hi % 8) == 0 && ((uintptr_t)lo % 8) == 0; }
   static constexpr bool kRowScale = false;
   __device__ __forceinline__ float row_scale(int64_t) const { return 1.f; }
 };
@@ -32,7 +43,14 @@ struct SplitBf16Vals {
 struct HiBf16Vals {
   const uint16_t* hi;
   template <typename C> __device__ __forceinline__ C get(int64_t j) const { return (C)bf16_to_f32(hi[j]); }
-  static constexpr bool kVec = false;
+  static constexpr bool kVec = true;
+  template <typename C> __device__ __forceinline__ void get4(int64_t j, C (&o)[4]) const {
+    uint16_t h[4];
+    load4(hi, j, h);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = (C)bf16_to_f32(h[k]);
+  }
+  bool vec_aligned() const { return ((uintptr_t)hi % 8) == 0; }
   static constexpr bool kRowScale = false;
   __device__ __forceinline__ float row_scale(int64_t) const { return 1.f; }
 };
@@ -41,7 +59,14 @@ struct ScaledF16Vals {
   const uint16_t* h;
   const float* scale;
   template <typename C> __device__ __forceinline__ C get(int64_t j) const { return (C)f16_to_f32(h[j]); }
-  static constexpr bool kVec = false;
+  static constexpr bool kVec = true;
+  template <typename C> __device__ __forceinline__ void get4(int64_t j, C (&o)[4]) const {
+    uint16_t v[4];
+    load4(h, j, v);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = (C)f16_to_f32(v[k]);
+  }
+  bool vec_aligned() const { return ((uintptr_t)h % 8) == 0; }
   static constexpr bool kRowScale = true;
   __device__ __forceinline__ float row_scale(int64_t r) const { return scale[r]; }
 };
@@ -104,7 +129,7 @@ int launch_split_xg(const fs_csr* M, const VS& vs, const void* xg, void* y, doub
   const int64_t grid = spmv_fill_schedule(p, M, pl, 1);
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr_split: grid of %lld blocks too large", (long long)grid);
-  return spmv_launch_entry<Comp, false, true>(p, grid, st, "fs_spmv_csr_split");
+  return spmv_launch_entry<Comp, true, true>(p, grid, st, "fs_spmv_csr_split");
 }
 
 template <typename VS, typename P, int S>
@@ -124,7 +149,7 @@ int launch_split_s(const fs_csr* M, const VS& vs, const void* x, void* y, double
   if (grid > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_spmv_csr_split: grid of %lld blocks too large", (long long)grid);
   if (pl->tile_nnz > 0 && pl->tile_mode == FS_TILE_ENTRIES)
-    return spmv_launch_entry<Comp, false>(p, grid, st, "fs_spmv_csr_split");
+    return spmv_launch_entry<Comp, true>(p, grid, st, "fs_spmv_csr_split");
   if (pl->tile_nnz > 0) {
     if (tile_lds_bytes<double>(pl->tile_nnz) > 65536)
       return fail(FS_ERR_VALUE, "fs_spmv_csr_split: tile_nnz %lld needs more than 64 KB of LDS",

@@ -58,6 +58,11 @@ struct DirectVals {
   }
   static constexpr bool kRowScale = false;
   __device__ __forceinline__ float row_scale(int64_t) const { return 1.f; }
+  // whether get4's loads are aligned (host side)
+  bool vec_aligned() const {
+    const size_t va = 4 * sizeof(V) >= 16 ? 16 : 4 * sizeof(V);
+    return ((uintptr_t)v % va) == 0;
+  }
   // raw stored value and its conversion, for loads that must stay
   // unconverted until a whole batch is issued (accumulate_span)
   static constexpr bool kRaw = std::is_arithmetic<V>::value;
@@ -891,8 +896,7 @@ bool entry_vec_ok(const SpmvParams<VS, V, I, P>& p) {
   if constexpr (!VS::kVec) {
     return false;
   } else {
-    const size_t va = 4 * sizeof(V) >= 16 ? 16 : 4 * sizeof(V);
-    return ((uintptr_t)p.idx % 16) == 0 && ((uintptr_t)p.vals.v % va) == 0;
+    return ((uintptr_t)p.idx % 16) == 0 && p.vals.vec_aligned();
   }
 }
 

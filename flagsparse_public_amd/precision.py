@@ -14,9 +14,12 @@ x / y and fp32 products, and accumulates in fp32 only: every row sum is a
 compensated (TwoSum) fp32 sum -- lane accumulators, the tile roles' row sums
 and the carries of rows cut by tile boundaries, chunk partials of long rows,
 and the column-sliced combine -- so no fp64 arithmetic is used (fp64 is kept
-for float64 calls and the oracle check).  When x is larger than 8 MB the
-default schedule also uses the column-sliced layout (slicing.py) with the
-planes permuted into slice-major order.  The reference stores values only in
+for float64 calls and the oracle check).  The default schedule is the one
+the plain fp32 SpMV of the matrix would use (the packaged selector, else the
+fixed rule): for x far beyond the L2 that
+is the x-window layout (xwin.py: x gathered from LDS-staged windows, then the
+planes streamed in CSR order) or the column-sliced layout (slicing.py, planes
+permuted into slice-major order).  The reference stores values only in
 the call's dtype; these modes have no counterpart there and are measured
 beside the plain fp32 kernel.
 """
@@ -28,9 +31,11 @@ import time
 import torch
 
 from . import _lib
-from .rowsplit import RowSplitPlan, Schedule, default_spmv_schedule
+from .rowsplit import RowSplitPlan, Schedule
+from .schedule_choice import choose_schedule
 from .slicing import SlicedCsr
 from .sparse_operations.spmv_csr import _prepare_spmv_csr_matrix, _validate_spmv_x
+from .xwin import XWinCsr
 
 FORMATS = {
     "bf16x2": _lib.FS_STORE_BF16X2,
@@ -84,8 +89,9 @@ def prepare_spmv_csr_split(data, indices, indptr, shape, value_format="bf16x2", 
         raise ValueError(f"value_format must be one of {sorted(FORMATS)}")
     if torch.is_tensor(data) and data.dtype != torch.float32:
         raise TypeError("split-precision storage is for float32 matrices")
+    stats_out = []
     (data, kernel_indices, kernel_indptr, n_rows, n_cols, row_lengths,
-     max_row_nnz) = _prepare_spmv_csr_matrix(data, indices, indptr, shape)
+     max_row_nnz) = _prepare_spmv_csr_matrix(data, indices, indptr, shape, _stats_out=stats_out)
     if kernel_indices.dtype != torch.int32:
         if n_cols - 1 > 2**31 - 1:
             raise ValueError("split-precision storage needs column indices that fit int32")
@@ -99,9 +105,28 @@ def prepare_spmv_csr_split(data, indices, indptr, shape, value_format="bf16x2", 
              if value_format == "f16_rowscale" else None)
     csr = _lib.make_csr(data, kernel_indices, kernel_indptr, n_rows, n_cols)
     _lib.split_values(fmt, csr, hi, lo, scale, dev)
-    sch = schedule or default_spmv_schedule(n_rows, nnz, torch.float32, n_cols)
+    if schedule is None:
+        # the layout the plain fp32 SpMV of this matrix would use (the packaged
+        # selector on the device, else the fixed rule)
+        sch, _ = choose_schedule("spmv", None, data, n_rows, n_cols, stats_out[0][0])
+    else:
+        sch = schedule
     return PreparedCsrSpmvSplit(kernel_indices, kernel_indptr, (n_rows, n_cols), value_format,
                                 hi, lo, scale, row_lengths, max_row_nnz, sch)
+
+
+def _xwin_split(prepared, sch):
+    """The x-window layout of the matrix (columns and row pointers only: the
+    split planes are read in CSR order by the second pass)."""
+    key = ("xwin", sch.key())
+    xw = prepared._plans.get(key)
+    if xw is None:
+        xw = XWinCsr(None, prepared.kernel_indices, prepared.kernel_indptr, prepared.n_rows,
+                     prepared.n_cols, tile_nnz=sch.tile_nnz if sch.tile_nnz > 0 else 2048,
+                     window=sch.x_window if sch.x_window > 0 else None,
+                     value_dtype=torch.float32)
+        prepared._plans[key] = xw
+    return xw
 
 
 def _sliced_split(prepared, sch):
@@ -154,7 +179,13 @@ def flagsparse_spmv_csr_split(x, prepared, out=None, alpha=1.0, beta=0.0, return
         torch.cuda.synchronize()
         t0 = time.perf_counter()
     sch = schedule or prepared.schedule
-    if prepared.n_rows and sch.x_slice_cols > 0 and prepared.n_cols > sch.x_slice_cols:
+    if prepared.n_rows and sch.x_window != 0 and 0 < prepared.kernel_indices.numel() < 2**31:
+        xw = _xwin_split(prepared, sch)
+        _lib.xwin_gather(xw.layout, x, xw.xg, x.device)
+        _lib.spmv_split(prepared._csr, FORMATS[prepared.value_format], prepared.hi, prepared.lo,
+                        prepared.row_scale, xw.xg, y, alpha, beta,
+                        xw.plan(_lib.FS_ACC_NATIVE).struct(1), x.device)
+    elif prepared.n_rows and sch.x_slice_cols > 0 and prepared.n_cols > sch.x_slice_cols:
         sl = _sliced_split(prepared, sch)
         if sl.n_compact:
             _lib.spmv_split(sl.csr, FORMATS[prepared.value_format], sl.planes[0], sl.planes[1],

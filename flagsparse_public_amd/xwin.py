@@ -95,9 +95,14 @@ class XWinCsr:
     """The x-window layout of one CSR matrix, its gathered-x buffer and the
     second pass's entry-tile plans (one per accumulation mode)."""
 
-    def __init__(self, data, indices, indptr, n_rows, n_cols, tile_nnz=2048, window=None):
-        if data.dtype not in XWIN_DTYPES:
-            raise TypeError(f"x windows support {XWIN_DTYPES}, got {data.dtype}")
+    def __init__(self, data, indices, indptr, n_rows, n_cols, tile_nnz=2048, window=None,
+                 value_dtype=None):
+        """``data`` may be None when the values are stored elsewhere (split-
+        precision planes): the layout depends only on the columns and row
+        pointers, and x / xg have ``value_dtype``."""
+        value_dtype = data.dtype if data is not None else value_dtype
+        if value_dtype not in XWIN_DTYPES:
+            raise TypeError(f"x windows support {XWIN_DTYPES}, got {value_dtype}")
         dev = indices.device
         nnz = int(indices.numel())
         if nnz > 2**31 - 1:
@@ -105,22 +110,23 @@ class XWinCsr:
         self.data, self.indices, self.indptr = data, indices, indptr
         self.n_rows, self.n_cols, self.nnz = int(n_rows), int(n_cols), nnz
         self.device = dev
-        self.value_dtype = data.dtype
+        self.value_dtype = value_dtype
         self.tile_nnz = int(tile_nnz)
-        W = int(window) if window else default_window(data.dtype, dev)
+        W = int(window) if window else default_window(value_dtype, dev)
         if W <= 0 or W % 4 or W > 65532:
             raise ValueError(f"window must be a multiple of 4 in (0, 65532], got {W}")
         self.window = W
         self.n_windows = max(1, -(-self.n_cols // W))
         self.schedule = Schedule(tile_nnz=self.tile_nnz, tile_mode=_lib.FS_TILE_ENTRIES)
         self._plans = {}
-        base = RowSplitPlan(indptr, self.n_rows, data.dtype, self.schedule, _lib.FS_ACC_NATIVE)
+        base = RowSplitPlan(indptr, self.n_rows, value_dtype, self.schedule, _lib.FS_ACC_NATIVE)
         self._plans[_lib.FS_ACC_NATIVE] = base
         self.tile_entry = base.tile_entry
         self.n_tiles = base.n_tiles
         self._build(indices, base)
-        self.xg = torch.empty(max(nnz, 1), dtype=data.dtype, device=dev)
-        self.csr = _lib.make_csr(data, indices, indptr, self.n_rows, self.n_cols)
+        self.xg = torch.empty(max(nnz, 1), dtype=value_dtype, device=dev)
+        self.csr = (_lib.make_csr(data, indices, indptr, self.n_rows, self.n_cols)
+                    if data is not None else None)
 
     def _build(self, indices, base):
         dev, nnz, W, S = self.device, self.nnz, self.window, self.n_windows

@@ -134,9 +134,11 @@ def test_bf16x2_powerlaw_4m_full_size(orc):
 @pytest.mark.parametrize("sch", [
     None,                                                    # library schedule
     Schedule(tile_nnz=2048, tile_mode=1),                    # entry tiles, carries
+    Schedule(tile_nnz=4096, tile_mode=1),                    # 8-byte plane loads
     Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=4096),  # sliced, compensated combine
     Schedule(group_lanes=8, short_max=2048, chunk_nnz=8192, unroll=4, tile_nnz=2048),
     Schedule(group_lanes=4, short_max=64, chunk_nnz=512, unroll=4),
+    Schedule(tile_nnz=2048, tile_mode=1, x_window=1024),     # x windows, 9 of them
 ])
 def test_split_long_rows_are_compensated(fmt, sch):
     """A 60,001-entry row 2^13, then 60,000 terms 2^-13 (x = 1), all exact in
@@ -161,3 +163,25 @@ def test_split_long_rows_are_compensated(fmt, sch):
     # exact, up to one fp32 rounding of each column-slice partial (8 slices)
     assert abs(y[7] - (8192.0 + 60_000 * 2.0 ** -13)) <= 8 * 2.0 ** -11
     assert np.all(y[np.arange(n) != 7] == 0.5)
+
+
+@pytest.mark.parametrize("T", [1024, 2048, 4096])
+@pytest.mark.parametrize("fmt", ["bf16x2", "bf16", "f16_rowscale"])
+def test_split_x_windows_equal_entry_tiles(orc, fmt, T):
+    """Split storage over the x-window layout: the second pass reads the
+    planes in CSR order with the gathered x (8-byte plane loads for full,
+    aligned tiles), so with tiles of 2048 y equals the one-pass entry tiles
+    of 2048 bit for bit; every tile size meets the decoded-values bar."""
+    ip, cols, vals = _matrix(seed=41, n=60_000)
+    n = ip.size - 1
+    x = synthetic.dense_uniform(42, n)
+    args = (to_dev(vals), to_dev(cols), to_dev(ip), (n, n), fmt)
+    xw = prepare_spmv_csr_split(*args, schedule=Schedule(tile_nnz=T, tile_mode=1,
+                                                         x_window=8192))
+    y = flagsparse_spmv_csr_split(to_dev(x), xw)
+    if T == 2048:
+        ref = prepare_spmv_csr_split(*args, schedule=Schedule(tile_nnz=2048, tile_mode=1))
+        assert torch.equal(y, flagsparse_spmv_csr_split(to_dev(x), ref))
+    dec = decoded_values(xw).cpu().numpy()
+    assert row_rel_err(host64(y), orc.spmv(ip, cols, dec, x),
+                       orc.spmv_abs(ip, cols, dec, x)).max() <= 1e-5
