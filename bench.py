@@ -602,7 +602,7 @@ def sub_unprepared(d_spmv, prepared_us, reps=20):
     same tensors reuse it."""
     import numpy as np
     import torch
-    from flagsparse_public_amd import flagsparse_spmv_csr, prepare_spmv_csr, synthetic
+    from flagsparse_public_amd import _lib, flagsparse_spmv_csr, prepare_spmv_csr, synthetic
     from flagsparse_public_amd.sparse_operations.spmv_csr import (_UNPREPARED,
                                                                   _prepare_spmv_csr_matrix)
     dev = torch.device("cuda", 0)
@@ -631,9 +631,14 @@ def sub_unprepared(d_spmv, prepared_us, reps=20):
         call = lambda: flagsparse_spmv_csr(vals_d, cols_d, ip_d, x_d, (n, n), use_opt=True)
         first = wall(call, 1)
         again = wall(call, reps)
-        valid = wall(lambda: _prepare_spmv_csr_matrix(vals_d, cols_d, ip_d, (n, n)), reps)
+        valid = wall(lambda: _prepare_spmv_csr_matrix(vals_d, cols_d, ip_d, (n, n),
+                                                      _lengths=False), reps)
+        csr = _lib.make_csr(vals_d, cols_d, ip_d, n, n)
+        kern = _event_time(lambda: _lib.analyze_device(csr, dev), reps,
+                           torch.cuda.current_stream(dev))
         out[name] = {"first_call_us": round(first, 1), "repeat_call_us": round(again, 1),
                      "prepared_step_us": round(step_us, 2), "validation_pass_us": round(valid, 1),
+                     "validation_kernel_us": round(kern, 1),
                      "repeat_over_step_plus_validation": round(again / (step_us + valid), 3)}
     _UNPREPARED.clear()
     _log(f"unprepared calls: {out}")

@@ -261,11 +261,12 @@ def _transpose_csr_for_spmv(data, indices, indptr, shape):
 
 
 def _prepare_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy="auto",
-                             _stats_out=None):
+                             _stats_out=None, _lengths=True):
     """Validation with the reference's checks, order and messages
     (spmv_csr.py:497-551); the device-side checks run as one fs_csr_analyze
     pass.  Returns (data, indices, indptr, n_rows, n_cols, row_lengths,
-    max_row_nnz)."""
+    max_row_nnz); row_lengths is None when ``_lengths`` is false (a
+    validation-only pass)."""
     _normalize_spmv_index_fallback_policy(index_fallback_policy)
     if not all(torch.is_tensor(t) for t in (data, indices, indptr)):
         raise TypeError("data, indices, indptr must all be torch.Tensor")
@@ -302,7 +303,7 @@ def _prepare_spmv_csr_matrix(data, indices, indptr, shape, index_fallback_policy
         raise ValueError("indptr must be non-decreasing")
     if nnz > 0 and (stats[3] < 0 or stats[4] >= n_cols):
         raise IndexError("indices out of range for n_cols")
-    row_lengths = (indptr[1:] - indptr[:-1]).to(torch.int64)
+    row_lengths = (indptr[1:] - indptr[:-1]).to(torch.int64) if _lengths else None
     max_row_nnz = int(stats[5]) if n_rows > 0 else 0
     return data, indices, indptr, n_rows, n_cols, row_lengths, max_row_nnz
 
@@ -436,7 +437,7 @@ def _prepare_for_call(data, indices, indptr, shape, block_nnz, max_segments, op_
             stats = []
             _prepare_spmv_csr_matrix(data, indices, indptr, shape,
                                      index_fallback_policy=index_fallback_policy,
-                                     _stats_out=stats)
+                                     _stats_out=stats, _lengths=False)
             if stats[0][1] == prepared._stats:
                 return prepared
     prepared = prepare_spmv_csr(data, indices, indptr, shape, block_nnz=block_nnz,
