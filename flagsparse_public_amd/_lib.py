@@ -28,6 +28,7 @@ FS_ACC_NATIVE, FS_ACC_WIDE = 0, 1
 FS_STORE_BF16X2, FS_STORE_BF16, FS_STORE_F16_ROWSCALE = 1, 2, 3
 FS_STATS_LEN = 16
 FS_TILE_ROWS, FS_TILE_ENTRIES = 0, 1
+FS_XG_TILE_MAJOR, FS_XG_WINDOW_MAJOR = 0, 1
 FS_NOALLOC_SCRATCH_BYTES = 4 << 20  # fs_spmv_csr_noalloc's per-GPU partials scratch
 
 _COMPLEX32 = getattr(torch, "complex32", None) or getattr(torch, "chalf", None)
@@ -85,6 +86,11 @@ class FsPlan(ctypes.Structure):
         ("row_sums", ctypes.c_void_p),
         ("tile_entry", ctypes.c_void_p),
         ("x_perm", ctypes.c_void_p),
+        ("xg_tile_run", ctypes.c_void_p),
+        ("xg_batch_run", ctypes.c_void_p),
+        ("xg_batch_mask", ctypes.c_void_p),
+        ("xg_run_off", ctypes.c_void_p),
+        ("xg_n_runs", ctypes.c_int64),
     ]
 
 
@@ -134,6 +140,8 @@ class FsXwin(ctypes.Structure):
         ("part_window", ctypes.c_void_p),
         ("part_entry", ctypes.c_void_p),
         ("window_entry", ctypes.c_void_p),
+        ("order", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

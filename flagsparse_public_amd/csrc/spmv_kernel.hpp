@@ -102,6 +102,12 @@ struct SpmvParams {
   int n_groups;                // entry tiles: block groups (0: block b = tile b)
   const int64_t* group_tiles;  // [n_groups + 1]
   void* row_sums;              // entry tiles: raw row sums instead of y (or null)
+  // gathered x stored window-major: the run table of the tile-order positions
+  const int32_t* xg_tile_run;  // [n_tiles + 1] first run of each tile
+  const int32_t* xg_batch_run;
+  const uint64_t* xg_batch_mask;
+  const int32_t* xg_run_off;   // null: x is tile-major (one stream per tile)
+  int64_t xg_n_runs;
 };
 
 // Accumulator used for chunks: fp32-native chunked rows get compensation.
@@ -446,6 +452,11 @@ int64_t spmv_fill_schedule(SpmvParams<VS, V, I, P>& p, const fs_csr* M, const fs
   p.n_groups = pl->n_groups;
   p.group_tiles = pl->group_tiles;
   p.row_sums = pl->row_sums;
+  p.xg_tile_run = pl->xg_tile_run;
+  p.xg_batch_run = pl->xg_batch_run;
+  p.xg_batch_mask = pl->xg_batch_mask;
+  p.xg_run_off = pl->xg_run_off;
+  p.xg_n_runs = pl->xg_n_runs;
   if (pl->tile_nnz > 0 && pl->tile_mode == FS_TILE_ENTRIES) {
     if (pl->n_groups <= 0) return pl->n_tiles;
     return pl->group_blocks * pl->n_groups;
@@ -510,6 +521,24 @@ template <typename C, typename V> __host__ __device__ inline size_t entry_xg_off
 template <typename C, typename V> __host__ __device__ inline size_t entry_xg_lds_bytes(int64_t tile_nnz) {
   if constexpr (kXgAliased<C, V>) return entry_lds_bytes<C>(tile_nnz);
   return entry_xg_offset<C, V>(tile_nnz) + sizeof(V) * (size_t)tile_nnz;
+}
+// Window-major gathered x also stages the tile's run table (XgwLds: batch
+// records of T / 64 + 2 batches, T run offsets), which is dead once the
+// tile's x values are loaded: the run offsets go where the products go
+// (4 T <= sizeof(C) T bytes) and the batch records in the row-boundary
+// slots, neither of which is written before the products; a barrier after
+// the last read of the table lets the staged x values overwrite it.  No LDS
+// beyond the tile-major form's.
+template <typename C, typename V> __host__ __device__ inline size_t entry_xgw_lds_bytes(int64_t tile_nnz) {
+  return entry_xg_lds_bytes<C, V>(tile_nnz);
+}
+static_assert(sizeof(int32_t) * kBndSlots >= 12 * (4096 / 64 + 2),
+              "the row-boundary slots hold the batch records of a 4096-entry tile");
+// LDS of the entry kernel: XG 0 plain, 1 tile-major gathered x, 2 window-major.
+template <typename C, typename V> __host__ __device__ inline size_t entry_kernel_lds(int xg,
+                                                                                    int64_t tile_nnz) {
+  return xg == 2 ? entry_xgw_lds_bytes<C, V>(tile_nnz)
+                 : xg == 1 ? entry_xg_lds_bytes<C, V>(tile_nnz) : entry_lds_bytes<C>(tile_nnz);
 }
 
 // A finished row of the entry role: y = alpha * sum + beta * y, or, with
@@ -647,6 +676,7 @@ __device__ __forceinline__ void entry_gather(const SpmvParams<VS, V, I, P>& p,
 // value).  They are staged in LDS with coalesced loads -- all issued before
 // the first LDS store -- and then read through the permutation (clamped into
 // the tile, so a damaged permutation cannot leave it).
+
 template <int E, typename VS, typename V, typename I, typename P, typename C>
 __device__ __forceinline__ void entry_stage_xg(const SpmvParams<VS, V, I, P>& p,
                                                const EntryTile<I, C, E>& f, V* xt) {
@@ -673,6 +703,115 @@ __device__ __forceinline__ void entry_stage_xg(const SpmvParams<VS, V, I, P>& p,
       const int o = u * kBlock + tid;
       if (o < f.len) xt[o] = v[u];
     }
+  }
+}
+
+// Window-major gathered x (fs_plan.xg_run_off): tile position d = e0 + o
+// lies in run r = batch_run[d / 64] + (run starts in the batch at 1 .. d % 64)
+// and its value at x[d + run_off[r]].  The tile's batch records and the run
+// offsets of its runs [tile_run[t], tile_run[t + 1]) -- at most T / 64 + 1
+// batches and T runs -- are loaded into LDS with one round of loads (each
+// thread loads a clamped index, so no load waits behind a per-lane branch;
+// runs past the first kBlock, which take windows narrower than T / kBlock
+// entries per tile, follow in a loop); after a barrier every position's value
+// load is issued at once, and after a second one the values are stored.  The
+// table's loads are issued before the tile's entry loads and both barriers
+// order LDS only, so the entry loads stay in flight throughout.  Consecutive
+// positions of a run are consecutive in x, so a wavefront's loads coalesce
+// run by run.  Every index is clamped into its array.
+template <int T> struct XgwLds {
+  static constexpr int kBatches = T / kWave + 2;
+  uint64_t* mask;
+  int32_t* run;
+  int32_t* off;
+  __device__ __forceinline__ XgwLds(char* batches, char* offsets)
+      : mask(reinterpret_cast<uint64_t*>(batches)),
+        run(reinterpret_cast<int32_t*>(batches + sizeof(uint64_t) * kBatches)),
+        off(reinterpret_cast<int32_t*>(offsets)) {}
+};
+
+// A barrier that orders LDS only: outstanding global loads stay in flight
+// across it (__syncthreads waits for every outstanding load as well).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The tile's run-table loads, issued before the tile's entry loads so that
+// waiting for them does not wait for those.
+struct XgwFetch {
+  int64_t e0, q0, b0;
+  int len, nr, nb;
+  int32_t br, ro;
+  uint64_t bm;
+};
+
+template <int E, typename VS, typename V, typename I, typename P>
+__device__ __forceinline__ void entry_xgw_fetch(const SpmvParams<VS, V, I, P>& p, int64_t t,
+                                                XgwFetch& g) {
+  constexpr int T = E * kBlock;
+  const int tid = threadIdx.x;
+  g.e0 = p.tile_entry[t];
+  int64_t e1 = p.tile_entry[t + 1];
+  e1 = e1 > p.nnz ? p.nnz : e1;
+  const int64_t span = e1 - g.e0;
+  g.len = span <= 0 ? 0 : (span < T ? (int)span : T);  // as entry_fetch clamps it
+  if (g.len == 0) return;  // uniform: an empty tile reads nothing
+  const int64_t rmax = p.xg_n_runs - 1;
+  int64_t q0 = p.xg_tile_run[t], q1 = p.xg_tile_run[t + 1];
+  q0 = q0 < 0 ? 0 : (q0 > rmax ? rmax : q0);
+  q1 = q1 <= q0 ? q0 + 1 : (q1 > rmax + 1 ? rmax + 1 : q1);
+  g.q0 = q0;
+  g.nr = (int)(q1 - q0 < T ? q1 - q0 : T);
+  g.b0 = g.e0 >> 6;
+  g.nb = (int)(((g.e0 + g.len - 1) >> 6) - g.b0 + 1);  // <= T / 64 + 1
+  const int bi = tid < g.nb ? tid : g.nb - 1;
+  g.br = p.xg_batch_run[g.b0 + bi];
+  g.bm = p.xg_batch_mask[g.b0 + bi];
+  g.ro = p.xg_run_off[q0 + (tid < g.nr ? tid : g.nr - 1)];
+}
+
+// Stores the run table, then issues every position's x load (v); called
+// before the tile's entry loads, so those and these are in flight together.
+template <int E, typename VS, typename V, typename I, typename P>
+__device__ __forceinline__ void entry_xgw_issue(const SpmvParams<VS, V, I, P>& p,
+                                                const XgwFetch& g, const XgwLds<E * kBlock>& L,
+                                                V (&v)[E]) {
+  const int tid = threadIdx.x;
+  if (g.len == 0) return;  // uniform: an empty tile reads nothing
+  if (tid < g.nb) {
+    L.run[tid] = g.br;
+    L.mask[tid] = g.bm;
+  }
+  if (tid < g.nr) L.off[tid] = g.ro;
+  // a tile with more than kBlock runs (windows of few columns): the rest
+  for (int i = kBlock + tid; i < g.nr; i += kBlock) L.off[i] = p.xg_run_off[g.q0 + i];
+  lds_barrier();
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int o = u * kBlock + tid;
+    const int64_t d = g.e0 + (o < g.len ? o : 0);
+    const int b = (int)((d >> 6) - g.b0);
+    const uint64_t upto = (2ull << (d & 63)) - 2;  // bits 1 .. d % 64
+    int64_t r = (int64_t)L.run[b] + __builtin_popcountll(L.mask[b] & upto) - g.q0;
+    r = r < 0 ? 0 : (r >= g.nr ? g.nr - 1 : r);
+    int64_t src = d + L.off[r];
+    src = src < 0 ? 0 : (src >= p.nnz ? p.nnz - 1 : src);
+    v[u] = p.x[src];
+  }
+}
+
+// After every read of the run table, which xt may overlay: the x values to LDS.
+template <int E, typename V>
+__device__ __forceinline__ void entry_xgw_store(const XgwFetch& g, const V (&v)[E], V* xt) {
+  const int tid = threadIdx.x;
+  lds_barrier();
+  if (g.len == 0) return;
+#pragma unroll
+  for (int u = 0; u < E; ++u) {
+    const int o = u * kBlock + tid;
+    if (o < g.len) xt[o] = v[u];
   }
 }
 
@@ -812,7 +951,7 @@ template <typename C, typename S, int T> struct EntryLds {
 
 // One tile per block (block b takes tile b, or with block groups the b / G-th
 // tile of group b % G).
-template <typename A, int E, bool VEC, bool XG, typename VS, typename V, typename I, typename P>
+template <typename A, int E, bool VEC, int XG, typename VS, typename V, typename I, typename P>
 __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I, P> p) {
   using C = typename A::compute_t;
   using S = typename TileSumOf<A>::T;
@@ -826,11 +965,21 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_kernel(SpmvParams<VS, V, I,
     if (t >= p.group_tiles[g + 1] || t >= p.n_tiles) return;
   }
   EntryTile<I, C, E> f;
+  V* xt = reinterpret_cast<V*>(smem + entry_xg_offset<C, V>(E * kBlock));
+  XgwFetch g;
+  V xw[XG == 2 ? E : 1];
+  if constexpr (XG == 2) {  // run table, then the tile's x loads, then its entry loads
+    const XgwLds<E * kBlock> L(reinterpret_cast<char*>(lds.bnd), smem);
+    entry_xgw_fetch<E>(p, t, g);
+    entry_xgw_issue<E>(p, g, L, xw);
+  }
   entry_fetch<E, VEC>(p, t, f);
   C xv[E];
-  if constexpr (XG) {
-    V* xt = reinterpret_cast<V*>(smem + entry_xg_offset<C, V>(E * kBlock));
-    entry_stage_xg(p, f, xt);
+  if constexpr (XG != 0) {
+    if constexpr (XG == 2) {
+      entry_xgw_store<E>(g, xw, xt);
+    } else
+      entry_stage_xg(p, f, xt);
     __syncthreads();
     entry_gather_xg(f, xt, xv);
     if constexpr (kXgAliased<C, V>) __syncthreads();  // every read of xt before the products
@@ -883,11 +1032,24 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS,
   entry_write(p, r1 - 1, acc);
 }
 
-template <typename A, int E, bool VEC, bool XG, typename VS, typename V, typename I, typename P>
+template <typename A, int E, bool VEC, int XG, typename VS, typename V, typename I, typename P>
 void spmv_launch_entry_e(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st) {
   using C = typename A::compute_t;
-  const size_t lds = XG ? entry_xg_lds_bytes<C, V>(E * kBlock) : entry_lds_bytes<C>(E * kBlock);
+  const size_t lds = entry_kernel_lds<C, V>(XG, E * kBlock);
   spmv_entry_kernel<A, E, VEC, XG><<<dim3((unsigned)grid), dim3(kBlock), lds, st>>>(p);
+}
+
+// Tile-major (XG 1) or window-major (XG 2, p.xg_run_off set) gathered x.
+template <typename A, int E, bool VEC, typename VS, typename V, typename I, typename P>
+void spmv_launch_entry_g(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st) {
+  if (p.xg_run_off) spmv_launch_entry_e<A, E, VEC, 2>(p, grid, st);
+  else spmv_launch_entry_e<A, E, VEC, 1>(p, grid, st);
+}
+
+template <typename A, int E, bool VEC, bool XG, typename VS, typename V, typename I, typename P>
+void spmv_launch_entry_x(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st) {
+  if constexpr (XG) spmv_launch_entry_g<A, E, VEC>(p, grid, st);
+  else spmv_launch_entry_e<A, E, VEC, 0>(p, grid, st);
 }
 
 // Whether the col / value arrays allow the 16-byte loads of the entry role.
@@ -907,7 +1069,7 @@ template <typename A, bool HOT, bool XG = false, typename VS, typename V, typena
 int spmv_launch_entry(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_t st,
                       const char* who) {
   using C = typename A::compute_t;
-  const size_t lds = XG ? entry_xg_lds_bytes<C, V>(p.tile_nnz) : entry_lds_bytes<C>(p.tile_nnz);
+  const size_t lds = entry_kernel_lds<C, V>(XG ? (p.xg_run_off ? 2 : 1) : 0, p.tile_nnz);
   if (lds > 65536)
     return fail(FS_ERR_VALUE, "%s: entry tiles of %lld entries need more than 64 KB of LDS", who,
                 (long long)p.tile_nnz);
@@ -916,16 +1078,16 @@ int spmv_launch_entry(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream_
   bool done = false;
   if constexpr (HOT) {
     switch (p.tile_nnz) {
-      case 1024: vec ? spmv_launch_entry_e<A, 4, true, XG>(p, grid, st) : spmv_launch_entry_e<A, 4, false, XG>(p, grid, st); done = true; break;
-      case 2048: vec ? spmv_launch_entry_e<A, 8, true, XG>(p, grid, st) : spmv_launch_entry_e<A, 8, false, XG>(p, grid, st); done = true; break;
+      case 1024: vec ? spmv_launch_entry_x<A, 4, true, XG>(p, grid, st) : spmv_launch_entry_x<A, 4, false, XG>(p, grid, st); done = true; break;
+      case 2048: vec ? spmv_launch_entry_x<A, 8, true, XG>(p, grid, st) : spmv_launch_entry_x<A, 8, false, XG>(p, grid, st); done = true; break;
       case 4096:
-        vec ? spmv_launch_entry_e<A, 16, true, XG>(p, grid, st) : spmv_launch_entry_e<A, 16, false, XG>(p, grid, st);
+        vec ? spmv_launch_entry_x<A, 16, true, XG>(p, grid, st) : spmv_launch_entry_x<A, 16, false, XG>(p, grid, st);
         done = true;
         break;
       default: break;
     }
   } else if (p.tile_nnz == 2048) {
-    spmv_launch_entry_e<A, 8, false, XG>(p, grid, st);
+    spmv_launch_entry_x<A, 8, false, XG>(p, grid, st);
     done = true;
   }
   if (!done)

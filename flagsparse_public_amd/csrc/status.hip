@@ -72,6 +72,8 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
   // only the entry-tile role stores raw row sums or reads gathered x
   if (P->x_perm && !(P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES))
     return fail(FS_ERR_VALUE, "%s: x_perm is offered for entry tiles only", who);
+  if ((P->xg_run_off || P->xg_tile_run || P->xg_batch_run || P->xg_batch_mask) && !P->x_perm)
+    return fail(FS_ERR_VALUE, "%s: an xg run table is offered with x_perm only", who);
   if (P->row_sums && !(P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES))
     return fail(FS_ERR_VALUE, "%s: row_sums are offered for entry tiles only", who);
   if (P->tile_nnz > 0 && P->tile_mode == FS_TILE_ENTRIES) {
@@ -91,6 +93,10 @@ int check_plan(const fs_plan* P, const fs_csr* A, int64_t n_dense, const char* w
     if (P->x_perm && ((A->value_type != FS_F32 && A->value_type != FS_F64) || P->row_sums))
       return fail(FS_ERR_VALUE, "%s: gathered x (x_perm) is offered for fp32 / fp64 values "
                   "without row_sums", who);
+    if (P->xg_run_off && (!P->x_perm || !P->xg_tile_run || !P->xg_batch_run ||
+                          !P->xg_batch_mask || P->xg_n_runs <= 0 || A->nnz <= 0))
+      return fail(FS_ERR_VALUE, "%s: a window-major xg needs x_perm, xg_tile_run, "
+                  "xg_batch_run, xg_batch_mask and xg_n_runs > 0", who);
     const int64_t need = fs_tile_carry_bytes(A->value_type, P->n_tiles);
     if (!P->partials || P->partials_bytes < need)
       return fail(FS_ERR_VALUE, "%s: carry buffer too small (%lld < %lld bytes)", who,

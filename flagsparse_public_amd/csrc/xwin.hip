@@ -17,6 +17,13 @@
 // run of an entry is found without a search: batch b (entries 64b .. 64b+63)
 // records the run of its first entry and a 64-bit mask of the entries that
 // start a run.
+//
+// That is the tile-major xg.  Its stores land in runs of ~T / n_windows
+// entries, and neighbouring runs of one tile (neighbouring windows) are
+// written by different workgroups, so most cache lines of xg are written in
+// parts, at different times.  The window-major xg (FS_XG_WINDOW_MAJOR)
+// writes entry k to xg[k] -- whole lines, in order -- and leaves the run
+// lookup to the second pass's loads, which tolerate partial lines.
 #include <atomic>
 #include <type_traits>
 
@@ -42,7 +49,11 @@ template <typename V> struct XwParams {
   int32_t window;
 };
 
-// Stage x[c0 .. c0 + nc) (c0 a multiple of the window, itself a multiple of 4) in LDS.
+// Stage x[c0 .. c0 + nc) (c0 a multiple of the window, itself a multiple of 4)
+// in LDS: kStageLoads 16-byte loads per thread issued before any is stored
+// (a 128 KB window is one round of loads; every part's waves wait for it).
+constexpr int kStageLoads = 8;
+
 template <typename V>
 __device__ __forceinline__ void stage_window(const V* __restrict__ x, int64_t c0, int64_t nc, V* xs) {
   const int tid = threadIdx.x;
@@ -52,7 +63,19 @@ __device__ __forceinline__ void stage_window(const V* __restrict__ x, int64_t c0
     const int64_t nv = nc / kPer;
     const Vec* src = reinterpret_cast<const Vec*>(x + c0);
     Vec* dst = reinterpret_cast<Vec*>(xs);
-    for (int64_t i = tid; i < nv; i += kXwBlock) dst[i] = src[i];
+    for (int64_t i0 = tid; i0 < nv; i0 += (int64_t)kXwBlock * kStageLoads) {
+      Vec v[kStageLoads];
+#pragma unroll
+      for (int u = 0; u < kStageLoads; ++u) {
+        const int64_t i = i0 + (int64_t)u * kXwBlock;
+        v[u] = src[i < nv ? i : i0];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageLoads; ++u) {
+        const int64_t i = i0 + (int64_t)u * kXwBlock;
+        if (i < nv) dst[i] = v[u];
+      }
+    }
     for (int64_t i = kPer * nv + tid; i < nc; i += kXwBlock) xs[i] = x[c0 + i];
   } else {
     for (int64_t i = tid; i < nc; i += kXwBlock) xs[i] = x[c0 + i];
@@ -136,6 +159,95 @@ __global__ __launch_bounds__(kXwBlock) void xwin_gather_kernel(XwParams<V> p) {
   }
 }
 
+// Window-major xg: entry k's value goes to xg[k], so a part's range is one
+// stream.  Each thread takes 16 / sizeof(V) consecutive entries per step --
+// one load of their in-window columns, LDS reads, one 16-byte store, so a
+// wavefront's stores are 1 KB contiguous -- with kWmSteps steps' loads
+// issued before the first is used; the unaligned head and tail of each
+// window's range (fewer than 8 entries each) go one entry per thread.
+// In-window columns are clamped into the window.
+constexpr int kWmSteps = 8;
+
+template <typename V> struct WmCols;  // in-window columns of one step
+template <> struct WmCols<float> { using T = uint2; };      // 4 x uint16
+template <> struct WmCols<double> { using T = uint32_t; };  // 2 x uint16
+
+template <typename V>
+__device__ __forceinline__ void wm_unpack(const typename WmCols<V>::T& w, int (&c)[16 / sizeof(V)]) {
+  if constexpr (sizeof(V) == 4) {
+    c[0] = (int)(w.x & 0xffffu);
+    c[1] = (int)(w.x >> 16);
+    c[2] = (int)(w.y & 0xffffu);
+    c[3] = (int)(w.y >> 16);
+  } else {
+    c[0] = (int)(w & 0xffffu);
+    c[1] = (int)(w >> 16);
+  }
+}
+
+template <typename V>
+__global__ __launch_bounds__(kXwBlock) void xwin_gather_wm_kernel(XwParams<V> p) {
+  extern __shared__ __align__(16) char smem[];
+  V* xs = reinterpret_cast<V*>(smem);
+  const int64_t part = blockIdx.x;
+  int64_t w = p.part_window[part];
+  w = w < 0 ? 0 : (w >= p.n_windows ? p.n_windows - 1 : w);
+  int64_t k0 = p.part_entry[part], kend = p.part_entry[part + 1];
+  k0 = k0 < 0 ? 0 : k0;
+  kend = kend > p.nnz ? p.nnz : kend;
+  const int tid = threadIdx.x;
+  const int wmax = p.window - 1;
+  for (; k0 < kend && w < p.n_windows; ++w) {
+    int64_t k1 = p.window_entry[w + 1];
+    k1 = k1 > kend ? kend : k1;
+    if (k1 <= k0) continue;  // a window without entries in this range (uniform)
+    const int64_t c0 = w * (int64_t)p.window;
+    int64_t nc = p.n_cols - c0;
+    nc = nc > p.window ? p.window : (nc < 0 ? 0 : nc);
+    __syncthreads();  // every read of the previous window is done
+    stage_window(p.x, c0, nc, xs);
+    __syncthreads();
+    // [kb, ke): the 8-aligned body; [k0, kb) and [ke, k1) one entry per thread
+    int64_t kb = (k0 + 7) & ~(int64_t)7, ke = k1 & ~(int64_t)7;
+    if (kb > ke) kb = ke = k1;
+    if (tid < kb - k0) {
+      const int c = p.wcol[k0 + tid];
+      p.xg[k0 + tid] = xs[c > wmax ? wmax : c];
+    }
+    if (tid < k1 - ke) {
+      const int c = p.wcol[ke + tid];
+      p.xg[ke + tid] = xs[c > wmax ? wmax : c];
+    }
+    constexpr int kPer = 16 / sizeof(V);
+    using Vec = typename std::conditional<sizeof(V) == 4, float4, double2>::type;
+    using Cw = typename WmCols<V>::T;
+    const int64_t ns = (ke - kb) / kPer;
+    const Cw* src = reinterpret_cast<const Cw*>(p.wcol + kb);
+    Vec* dst = reinterpret_cast<Vec*>(p.xg + kb);
+    for (int64_t g0 = tid; g0 < ns; g0 += (int64_t)kXwBlock * kWmSteps) {
+      Cw cw[kWmSteps];
+#pragma unroll
+      for (int u = 0; u < kWmSteps; ++u) {
+        const int64_t g = g0 + (int64_t)u * kXwBlock;
+        cw[u] = src[g < ns ? g : g0];
+      }
+#pragma unroll
+      for (int u = 0; u < kWmSteps; ++u) {
+        const int64_t g = g0 + (int64_t)u * kXwBlock;
+        if (g >= ns) break;
+        int c[kPer];
+        wm_unpack<V>(cw[u], c);
+        Vec out;
+        V* o = reinterpret_cast<V*>(&out);
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) o[i] = xs[c[i] > wmax ? wmax : c[i]];
+        dst[g] = out;
+      }
+    }
+    k0 = k1;
+  }
+}
+
 template <typename V>
 int launch_xwin(const fs_xwin* L, const void* x, void* xg, hipStream_t st) {
   XwParams<V> p;
@@ -168,12 +280,18 @@ int launch_xwin(const fs_xwin* L, const void* x, void* xg, hipStream_t st) {
       if ((size_t)most < lds) most = (int)lds;
       hipError_t e = hipFuncSetAttribute((const void*)xwin_gather_kernel<V>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, most);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)xwin_gather_wm_kernel<V>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, most);
       if (e != hipSuccess)
         return fail(FS_ERR_HIP, "fs_xwin_gather: hipFuncSetAttribute: %s", hipGetErrorString(e));
       done.store(true, std::memory_order_release);
     }
   }
-  xwin_gather_kernel<V><<<dim3((unsigned)L->n_parts), dim3(kXwBlock), lds, st>>>(p);
+  if (L->order == FS_XG_WINDOW_MAJOR)
+    xwin_gather_wm_kernel<V><<<dim3((unsigned)L->n_parts), dim3(kXwBlock), lds, st>>>(p);
+  else
+    xwin_gather_kernel<V><<<dim3((unsigned)L->n_parts), dim3(kXwBlock), lds, st>>>(p);
   return check_launch("fs_xwin_gather");
 }
 
@@ -208,14 +326,21 @@ extern "C" int fs_xwin_gather(const fs_xwin* L, const void* x, void* xg, void* s
     return fail(FS_ERR_UNSUPPORTED, "fs_xwin_gather: more than 2^31 - 1 entries");
   if (L->window <= 0 || L->window % 4 != 0)
     return fail(FS_ERR_VALUE, "fs_xwin_gather: window must be a positive multiple of 4");
+  if (L->order != FS_XG_TILE_MAJOR && L->order != FS_XG_WINDOW_MAJOR)
+    return fail(FS_ERR_VALUE, "fs_xwin_gather: order must be FS_XG_TILE_MAJOR or "
+                "FS_XG_WINDOW_MAJOR");
   if (L->nnz == 0 || L->n_parts == 0) return FS_OK;
-  if (L->n_runs == 0 || L->n_cols == 0)
+  const bool runs = L->order == FS_XG_TILE_MAJOR;  // only the tile-major xg reads runs
+  if ((runs && L->n_runs == 0) || L->n_cols == 0)
     return fail(FS_ERR_VALUE, "fs_xwin_gather: entries without runs or columns");
-  if (!x || !xg || !L->wcol || !L->batch_run || !L->batch_mask || !L->run_off ||
-      !L->part_window || !L->part_entry || !L->window_entry)
+  if (!x || !xg || !L->wcol || !L->part_window || !L->part_entry || !L->window_entry ||
+      (runs && (!L->batch_run || !L->batch_mask || !L->run_off)))
     return fail(FS_ERR_VALUE, "fs_xwin_gather: null array");
   if (L->n_parts > 0x7fffffffLL)
     return fail(FS_ERR_UNSUPPORTED, "fs_xwin_gather: too many parts");
+  if (!runs && (((uintptr_t)L->wcol | (uintptr_t)xg) & 15) != 0)
+    return fail(FS_ERR_VALUE, "fs_xwin_gather: a window-major xg needs wcol and xg 16-byte "
+                "aligned");
   const int64_t max_w = fs_xwin_max_window(L->value_type);
   if (L->window > max_w)
     return fail(FS_ERR_UNSUPPORTED, "fs_xwin_gather: window of %d columns exceeds this "

@@ -138,6 +138,7 @@ class RowSplitPlan:
         self.n_groups, self.group_blocks, self.group_tiles = 0, 0, None
         self.row_sums = None
         self.x_perm = None
+        self.xg_runs = None
         if schedule.tile_nnz > 0 and schedule.tile_mode == _lib.FS_TILE_ENTRIES:
             self._init_entry_tiles(indptr, n_rows, schedule)
             return
@@ -228,10 +229,13 @@ class RowSplitPlan:
         self.row_sums = sums
         self._structs.clear()
 
-    def set_x_perm(self, perm: torch.Tensor):
+    def set_x_perm(self, perm: torch.Tensor, runs=None):
         """Entry tiles: x is gathered per entry (x windows); entry j of tile t
-        reads x[tile_entry[t] + perm[j]]."""
+        reads tile position d = tile_entry[t] + perm[j], at x[d] (tile-major
+        x) or, with ``runs`` = (tile_run, batch_run, batch_mask, run_off) of
+        the tile positions, at x[d + run_off[run of d]] (window-major x)."""
         self.x_perm = perm
+        self.xg_runs = runs
         self._structs.clear()
 
     def struct(self, n_dense: int = 1) -> _lib.FsPlan:
@@ -260,6 +264,13 @@ class RowSplitPlan:
                              row_sums=_lib.ptr(self.row_sums),
                              tile_entry=_lib.ptr(self.tile_entry),
                              x_perm=_lib.ptr(self.x_perm))
+            runs = self.xg_runs
+            if runs is not None:
+                st.xg_tile_run = _lib.ptr(runs[0])
+                st.xg_batch_run = _lib.ptr(runs[1])
+                st.xg_batch_mask = _lib.ptr(runs[2])
+                st.xg_run_off = _lib.ptr(runs[3])
+                st.xg_n_runs = int(runs[3].numel())
             self._structs[key] = st
             return st
         if self.n_long:

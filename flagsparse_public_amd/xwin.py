@@ -34,6 +34,18 @@ Layout (all on the device, built once per matrix by stable sorts):
 Bytes per entry (fp32): pass 1 reads 2 (wcol) and writes 4 (xg); pass 2 reads
 4 (value) + 4 (xg) + 2 (x_perm).  16 bytes, all in streams, against 12 in the
 one-pass kernel of which 4 are random gathers.
+
+``order`` places xg in one of two orders:
+
+* ``"tile"`` (tile-major): xg is in the second pass's order, so pass 2 reads
+  each tile's values as one stream, and pass 1 writes each run to its place
+  -- runs of ~T / n_windows entries, neighbouring ones written by different
+  workgroups, so most cache lines of xg are written in parts.
+* ``"window"`` (window-major): pass 1 writes xg[k] for window-major entry k,
+  whole lines in order, and the run table (``batch_run`` / ``batch_mask`` /
+  ``run_off``, now over the tile-order positions, with ``tile_run`` = the
+  first run of each tile) moves to pass 2, whose loads find tile position d
+  at xg[d + run_off[run of d]].
 """
 
 from __future__ import annotations
@@ -47,6 +59,8 @@ from .rowsplit import RowSplitPlan, Schedule
 
 XWIN_DTYPES = (torch.float32, torch.float64)
 DEFAULT_WINDOW_BYTES = 128 * 1024
+ORDERS = {"tile": _lib.FS_XG_TILE_MAJOR, "window": _lib.FS_XG_WINDOW_MAJOR}
+N_XCDS = 8  # MI355X: workgroups are dealt to 8 XCDs round-robin
 
 
 def default_window(value_dtype, device) -> int:
@@ -62,6 +76,26 @@ def default_window(value_dtype, device) -> int:
 def _as_int16(t: torch.Tensor) -> torch.Tensor:
     """int64 values in [0, 65536) stored as their 16-bit pattern."""
     return torch.where(t >= 32768, t - 65536, t).to(torch.int16)
+
+
+def run_table(is_start: torch.Tensor, off_at_start: torch.Tensor):
+    """(batch_run, batch_mask, run_off) of a sequence cut into runs:
+    ``is_start[i]`` marks the first position of each run (position 0
+    included), ``off_at_start`` holds each run's offset, in run order.
+    batch_run[b] = run of position 64 b; bit i >= 1 of batch_mask[b] is set
+    when position 64 b + i starts a run."""
+    dev, n = is_start.device, is_start.numel()
+    nb = -(-n // 64)
+    run_id = torch.cumsum(is_start.to(torch.int64), 0) - 1
+    batch_run = run_id[torch.arange(nb, device=dev, dtype=torch.int64) * 64] \
+        .to(torch.int32).contiguous()
+    del run_id
+    pad = torch.zeros(nb * 64, dtype=torch.int64, device=dev)
+    pad[:n] = is_start.to(torch.int64)
+    bits = torch.bitwise_left_shift(pad.view(nb, 64),
+                                     torch.arange(64, device=dev, dtype=torch.int64))
+    batch_mask = bits.sum(dim=1).contiguous()  # distinct bits: the sum is the OR
+    return batch_run, batch_mask, off_at_start.to(torch.int32).contiguous()
 
 
 def gather_parts(device, window_bytes: int) -> int:
@@ -96,10 +130,14 @@ class XWinCsr:
     second pass's entry-tile plans (one per accumulation mode)."""
 
     def __init__(self, data, indices, indptr, n_rows, n_cols, tile_nnz=2048, window=None,
-                 value_dtype=None):
+                 value_dtype=None, order="window"):
         """``data`` may be None when the values are stored elsewhere (split-
         precision planes): the layout depends only on the columns and row
-        pointers, and x / xg have ``value_dtype``."""
+        pointers, and x / xg have ``value_dtype``.  ``order``: "window" or
+        "tile" (see the module doc)."""
+        if order not in ORDERS:
+            raise ValueError(f"order must be one of {sorted(ORDERS)}, got {order!r}")
+        self.order = order
         value_dtype = data.dtype if data is not None else value_dtype
         if value_dtype not in XWIN_DTYPES:
             raise TypeError(f"x windows support {XWIN_DTYPES}, got {value_dtype}")
@@ -132,6 +170,7 @@ class XWinCsr:
         dev, nnz, W, S = self.device, self.nnz, self.window, self.n_windows
         n_tiles = self.n_tiles
         te = self.tile_entry
+        self.xg_runs = None
         if nnz == 0:
             z16 = torch.zeros(1, dtype=torch.int16, device=dev)
             self.wcol, self.x_perm = z16, z16
@@ -155,34 +194,46 @@ class XWinCsr:
         self.wcol = _as_int16((cols - win * W)[order1]).contiguous()
         del cols
         # pass-2 layout: by (tile, window, CSR order)
-        order2 = torch.argsort(tile * S + win, stable=True)
+        key2 = tile * S + win
+        order2 = torch.argsort(key2, stable=True)
         pos2 = torch.empty(nnz, dtype=torch.int64, device=dev)
         pos2[order2] = torch.arange(nnz, device=dev, dtype=torch.int64)
-        del order2
         self.x_perm = _as_int16(pos2 - te[tile]).contiguous()
-        # runs: (window, tile) groups in window-major order
-        key1 = (win * n_tiles + tile)[order1]
-        del tile
-        is_start = torch.ones(nnz, dtype=torch.bool, device=dev)
-        is_start[1:] = key1[1:] != key1[:-1]
-        del key1
-        run_k0 = torch.nonzero(is_start, as_tuple=False).flatten()
-        self.n_runs = int(run_k0.numel())
-        dst = pos2[order1]
-        del pos2
-        self.run_off = (dst[run_k0] - run_k0).to(torch.int32).contiguous()
-        del dst
-        nb = -(-nnz // 64)
-        run_id = torch.cumsum(is_start.to(torch.int64), 0) - 1
-        self.batch_run = run_id[torch.arange(nb, device=dev, dtype=torch.int64) * 64] \
-            .to(torch.int32).contiguous()
-        del run_id
-        pad = torch.zeros(nb * 64, dtype=torch.int64, device=dev)
-        pad[:nnz] = is_start.to(torch.int64)
-        bits = torch.bitwise_left_shift(pad.view(nb, 64),
-                                         torch.arange(64, device=dev, dtype=torch.int64))
-        self.batch_mask = bits.sum(dim=1).contiguous()  # distinct bits: the sum is the OR
-        del pad, bits, is_start
+        # runs: the (window, tile) groups, contiguous in both orders
+        if self.order == "tile":
+            # over the window-major entries k: entry k goes to xg[k + run_off]
+            del order2, key2
+            key1 = (win * n_tiles + tile)[order1]
+            del tile
+            is_start = torch.ones(nnz, dtype=torch.bool, device=dev)
+            is_start[1:] = key1[1:] != key1[:-1]
+            del key1
+            run_k0 = torch.nonzero(is_start, as_tuple=False).flatten()
+            dst = pos2[order1]
+            del pos2
+            self.batch_run, self.batch_mask, self.run_off = run_table(
+                is_start, dst[run_k0] - run_k0)
+            del dst, is_start
+        else:
+            # over the tile-order positions d: position d reads xg[d + run_off]
+            del tile, pos2
+            key2 = key2[order2]
+            is_start = torch.ones(nnz, dtype=torch.bool, device=dev)
+            is_start[1:] = key2[1:] != key2[:-1]
+            del key2
+            run_d0 = torch.nonzero(is_start, as_tuple=False).flatten()
+            pos1 = torch.empty(nnz, dtype=torch.int64, device=dev)
+            pos1[order1] = torch.arange(nnz, device=dev, dtype=torch.int64)
+            src = pos1[order2[run_d0]]
+            del pos1, order2
+            self.batch_run, self.batch_mask, self.run_off = run_table(is_start, src - run_d0)
+            del src, is_start
+            # tile t's runs: those starting in [te[t], te[t+1]) (runs never
+            # cross a tile boundary)
+            self.tile_run = torch.searchsorted(run_d0, te).to(torch.int32).contiguous()
+            del run_d0
+            self.xg_runs = (self.tile_run, self.batch_run, self.batch_mask, self.run_off)
+        self.n_runs = int(self.run_off.numel())
         # window ranges in window-major order -> gather parts
         win_sorted = win[order1]
         ws = torch.searchsorted(win_sorted, torch.arange(S + 1, device=dev, dtype=torch.int64))
@@ -203,7 +254,8 @@ class XWinCsr:
                            batch_mask=_lib.ptr(self.batch_mask), run_off=_lib.ptr(self.run_off),
                            part_window=_lib.ptr(self.part_window),
                            part_entry=_lib.ptr(self.part_entry),
-                           window_entry=_lib.ptr(self.window_entry))
+                           window_entry=_lib.ptr(self.window_entry),
+                           order=ORDERS[self.order], reserved=0)
 
     def plan(self, accumulate: int) -> RowSplitPlan:
         p = self._plans.get(int(accumulate))
@@ -212,7 +264,15 @@ class XWinCsr:
                              accumulate)
             self._plans[int(accumulate)] = p
         if p.x_perm is None:
-            p.set_x_perm(self.x_perm)
+            p.set_x_perm(self.x_perm, self.xg_runs)
+            if self.xg_runs is not None and self.n_tiles >= 2 * N_XCDS:
+                # neighbouring tiles read the two ends of shared cache lines
+                # of xg (a tile's run in a window ends where the next tile's
+                # begins): deal each XCD a contiguous tile range, so both
+                # reads of a line go through one L2
+                gt = torch.tensor([self.n_tiles * g // N_XCDS for g in range(N_XCDS + 1)],
+                                  dtype=torch.int64, device=self.device)
+                p.set_groups(N_XCDS, -(-self.n_tiles // N_XCDS), gt)
         return p
 
     def run(self, x, y, alpha, beta, accumulate, device):
@@ -227,6 +287,8 @@ class XWinCsr:
         """Device bytes this layout adds to the matrix."""
         t = (self.wcol, self.x_perm, self.batch_run, self.batch_mask, self.run_off,
              self.part_window, self.part_entry, self.window_entry, self.xg)
+        if self.xg_runs is not None:
+            t = t + (self.tile_run,)
         return int(sum(a.numel() * a.element_size() for a in t))
 
 

@@ -125,6 +125,19 @@ typedef struct fs_plan {
   void *row_sums;                /* FS_TILE_ENTRIES: if set, raw row sums  */
   const int64_t *tile_entry;     /* FS_TILE_ENTRIES: [n_tiles + 1]         */
   const uint16_t *x_perm;        /* FS_TILE_ENTRIES: if set, x is gathered */
+  /* x_perm over a window-major xg (fs_xwin.order = FS_XG_WINDOW_MAJOR): the
+     run table that finds each tile-order position's value in xg; all null
+     (and xg_n_runs 0) for a tile-major xg.  Tile-order position d (the
+     entries sorted by tile, then window, then CSR order) lies in run r =
+     xg_batch_run[d / 64] + (set bits of xg_batch_mask[d / 64] at 1 .. d % 64)
+     and takes xg[d + xg_run_off[r]]; tile t's positions lie in its runs
+     xg_tile_run[t] .. xg_tile_run[t + 1] - 1 (at most tile_nnz of them). */
+  const int32_t *xg_tile_run;    /* [n_tiles + 1]                          */
+  const int32_t *xg_batch_run;   /* [ceil(nnz / 64)]                       */
+  const uint64_t *xg_batch_mask; /* [ceil(nnz / 64)] bit i (i >= 1) set when
+                                    position 64 b + i starts a run         */
+  const int32_t *xg_run_off;     /* [xg_n_runs]                            */
+  int64_t xg_n_runs;
 } fs_plan;
 
 /*
@@ -161,7 +174,9 @@ typedef struct fs_plan {
  *                    lie at x[tile_entry[t] .. tile_entry[t+1]), in the
  *                    order x_perm gives -- entry j of tile t takes
  *                    x[tile_entry[t] + x_perm[j]] -- and col_idx is not read
- *                    (tile_nnz 1024, 2048 or 4096).
+ *                    (tile_nnz 1024, 2048 or 4096).  With xg_run_off also
+ *                    set, x is window-major: tile position d = tile_entry[t]
+ *                    + x_perm[j] is found at x[d + xg_run_off[run of d]].
  */
 enum { FS_TILE_ROWS = 0, FS_TILE_ENTRIES = 1 };
 
@@ -261,8 +276,9 @@ int fs_sliced_combine(const fs_slices *sl, const void *z, void *y, double alpha,
  * run is the entries of one entry tile inside one window.  fs_xwin_gather
  * stages each window of x in one workgroup's LDS and writes every entry's x
  * value to xg: entry k (window-major position) of run r goes to
- * xg[k + run_off[r]].  The second pass is fs_spmv_csr with entry tiles and
- * fs_plan.x_perm.
+ * xg[k + run_off[r]] (tile-major xg), or to xg[k] (window-major xg).  The
+ * second pass is fs_spmv_csr with entry tiles and fs_plan.x_perm (and, for a
+ * window-major xg, fs_plan.xg_run_off).
  */
 typedef struct fs_xwin {
   int32_t value_type;           /* of x and xg: FS_F32 or FS_F64                */
@@ -283,7 +299,18 @@ typedef struct fs_xwin {
                                    may run through several windows             */
   const int64_t *window_entry;  /* [ceil(n_cols / window) + 1] first
                                    window-major entry of each window          */
+  int32_t order;                /* FS_XG_TILE_MAJOR: entry k goes to
+                                   xg[k + run_off[r]] (tile order, read by the
+                                   second pass as one stream per tile);
+                                   FS_XG_WINDOW_MAJOR: entry k goes to xg[k]
+                                   (batch_run / batch_mask / run_off unused; the
+                                   second pass finds its values through
+                                   fs_plan.xg_run_off; wcol and xg 16-byte
+                                   aligned)                                    */
+  int32_t reserved;
 } fs_xwin;
+
+enum { FS_XG_TILE_MAJOR = 0, FS_XG_WINDOW_MAJOR = 1 };
 
 /* Largest window (columns) one workgroup can stage on the current device, or -1. */
 int64_t fs_xwin_max_window(int32_t value_type);

@@ -12,8 +12,9 @@ import numpy as np
 import pytest
 import torch
 
-from flagsparse_public_amd import flagsparse_spmv_csr, prepare_spmv_csr, synthetic
+from flagsparse_public_amd import _lib, flagsparse_spmv_csr, prepare_spmv_csr, synthetic
 from flagsparse_public_amd.rowsplit import Schedule
+from flagsparse_public_amd.xwin import XWinCsr
 from tests._util import host64, row_rel_err, to_dev, ulp_distance_f32
 
 pytestmark = pytest.mark.gpu
@@ -80,6 +81,30 @@ def test_xwin_equals_entry_tiles(orc, kind, dtype, use_opt, T, window):
     assert torch.equal(y2, y)
     y64, cond = _oracle(orc, ip, cols, vals, x)
     _check(y, y64, cond, dtype, use_opt)
+
+
+@pytest.mark.parametrize("dtype,T", [(torch.float32, 1024), (torch.float32, 2048),
+                                     (torch.float32, 4096), (torch.float64, 2048)])
+@pytest.mark.parametrize("kind", ["powerlaw", "poisson"])
+@pytest.mark.parametrize("window", [128, 4096, None])
+def test_xg_orders_equal_entry_tiles(kind, dtype, T, window):
+    """Tile-major and window-major xg give the one-pass entry tiles' result
+    bit for bit (windows of 128 columns give power-law tiles of 2048 or 4096
+    entries more than 1024 runs, which the staging loads in a loop)."""
+    ip, cols, vals, n_cols = _matrix(kind, dtype)
+    n = ip.size - 1
+    print(f"[xwin-order] {kind} {dtype} T={T} W={window}", flush=True)
+    x = to_dev(synthetic.dense_uniform(8, n_cols, dtype=vals.dtype))
+    args = [to_dev(vals), to_dev(cols), to_dev(ip)]
+    ref = prepare_spmv_csr(*args, (n, n_cols), schedule=Schedule(tile_nnz=T, tile_mode=1))
+    y_ref = flagsparse_spmv_csr(x=x, prepared=ref, use_opt=True)
+    for order in ("tile", "window"):
+        xw = XWinCsr(*args, n, n_cols, tile_nnz=T, window=window, order=order)
+        if window == 128 and order == "window" and kind == "powerlaw" and T >= 2048:
+            assert int((xw.tile_run[1:] - xw.tile_run[:-1]).max()) > 1024
+        y = torch.empty_like(y_ref)
+        xw.run(x, y, 1.0, 0.0, _lib.FS_ACC_NATIVE, x.device)
+        assert torch.equal(y, y_ref), order
 
 
 def test_xwin_alpha_beta_int64_and_misaligned_x(orc):

@@ -1,12 +1,15 @@
 """The x-window layout, checked on the CPU by replaying both passes in numpy.
 
-Pass 1 as the gather kernel runs it: entry k (window-major) finds its run from
-its batch's first run plus the popcount of the batch's run-start bits below
-it, and writes x[window * W + wcol[k]] to xg[k + run_off[run]].  Pass 2 as the
-entry kernel reads it: CSR entry j of tile t takes xg[tile_entry[t] +
-x_perm[j]].  With distinct x values, every entry must read exactly
-x[col[j]], the destinations must be a permutation, and each gather part must
-lie inside one window.  Then sum(val * xg) per row equals the oracle's SpMV.
+Tile-major xg, pass 1 as the gather kernel runs it: entry k (window-major)
+finds its run from its batch's first run plus the popcount of the batch's
+run-start bits below it, and writes x[window * W + wcol[k]] to
+xg[k + run_off[run]].  Window-major xg: pass 1 writes entry k to xg[k], and
+pass 2 finds tile position d's run the same way and reads xg[d + run_off[run]].
+Pass 2 as the entry kernel reads it: CSR entry j of tile t takes tile position
+tile_entry[t] + x_perm[j].  With distinct x values, every entry must read
+exactly x[col[j]], the destinations (sources) must be a permutation, and each
+gather part must start inside its recorded window.  Then sum(val * xg) per
+row equals the oracle's SpMV.
 """
 
 import numpy as np
@@ -34,20 +37,32 @@ def _replay(xw, x):
     below = (np.uint64(2) << lane) - np.uint64(2)  # lane 63: 2 << 63 wraps to 0
     r = br[k // 64] + np.bitwise_count(bm[k // 64] & below).astype(np.int64)
     assert r.min() >= 0 and r.max() < xw.n_runs
-    dst = k + off[r]
-    assert np.array_equal(np.sort(dst), k), "destinations are not a permutation"
+    moved = k + off[r]  # tile-major: destination of k; window-major: source of d = k
+    assert np.array_equal(np.sort(moved), k), "run offsets do not give a permutation"
     win = np.searchsorted(ws, k, side="right") - 1  # the window of each entry
-    xg = np.empty(nnz, dtype=x.dtype)
-    xg[dst] = x[win * xw.window + wcol]
-    return xg
+    gathered = x[win * xw.window + wcol]
+    if xw.order == "tile":
+        xg = np.empty(nnz, dtype=x.dtype)
+        xg[moved] = gathered
+        return xg  # already in tile order
+    # pass 2 stages tile t from its own runs tile_run[t] .. tile_run[t+1]-1,
+    # at most T of them
+    te = xw.tile_entry.numpy()
+    tr = xw.tile_run.numpy().astype(np.int64)
+    assert tr[0] == 0 and tr[-1] == xw.n_runs and np.all(np.diff(tr) >= 0)
+    assert np.all(np.diff(tr) <= xw.tile_nnz)
+    tile = np.repeat(np.arange(xw.n_tiles), np.diff(te))
+    assert np.all((tr[tile] <= r) & (r < tr[tile + 1]))
+    return gathered[moved]  # the tile-order values pass 2 stages
 
 
-def _check(ip, cols, vals, T, W, orc):
+def _check(ip, cols, vals, T, W, orc, order="window"):
     n_rows = ip.size - 1
     n_cols = int(cols.max()) + 1 if cols.size else 1
     n_cols = max(n_cols, 7 * W // 2 + 3)  # a last window narrower than W
     xw = XWinCsr(torch.from_numpy(vals), torch.from_numpy(cols), torch.from_numpy(ip),
-                 n_rows, n_cols, tile_nnz=T, window=W)
+                 n_rows, n_cols, tile_nnz=T, window=W, order=order)
+    assert (xw.xg_runs is not None) == (order == "window")
     x = np.arange(n_cols, dtype=np.float64) + 0.25  # distinct
     xg = _replay(xw, x) if xw.nnz else np.zeros(0)
     te = xw.tile_entry.numpy()
@@ -73,11 +88,27 @@ def _check(ip, cols, vals, T, W, orc):
     return xw
 
 
+@pytest.mark.parametrize("order", ["tile", "window"])
 @pytest.mark.parametrize("T", [1024, 2048])
 @pytest.mark.parametrize("W", [64, 1000, 4096])
-def test_powerlaw_layout(orc, T, W):
+def test_powerlaw_layout(orc, T, W, order):
     ip, cols, vals = synthetic.powerlaw_csr(6000, 5, alpha=1.8, max_len=5000)
-    _check(ip, cols, vals, T, W, orc)
+    _check(ip, cols, vals, T, W, orc, order)
+
+
+def test_row_capped_tiles_start_off_batch(orc):
+    # runs of one-entry rows cut tiles every 1024 rows at entry offsets that
+    # are not multiples of 64: pass 2's positions then straddle batches
+    lens = np.ones(9000, np.int64)
+    lens[::97] = 40
+    ip = np.zeros(lens.size + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    rng = np.random.default_rng(8)
+    cols = rng.integers(0, 20_000, int(ip[-1])).astype(np.int32)
+    vals = rng.standard_normal(cols.size).astype(np.float32)
+    for order in ("tile", "window"):
+        xw = _check(ip, cols, vals, 2048, 1000, orc, order)
+        assert np.any(xw.tile_entry.numpy()[1:-1] % 64 != 0)
 
 
 def test_banded_and_empty_rows(orc):
@@ -87,13 +118,15 @@ def test_banded_and_empty_rows(orc):
     keep = np.repeat(np.arange(lens.size), np.diff(ip)) % 3 != 0
     ip2 = np.zeros_like(ip)
     np.cumsum(lens, out=ip2[1:])
-    _check(ip2, cols[keep], vals[keep], 2048, 256, orc)
+    for order in ("tile", "window"):
+        _check(ip2, cols[keep], vals[keep], 2048, 256, orc, order)
 
 
 def test_int64_indices_and_single_window(orc):
     ip, cols, vals = synthetic.poisson_csr(3000, 9.0, 4, index_dtype=np.int64)
-    _check(ip.astype(np.int64), cols, vals, 1024, 4096, orc)
-    _check(ip.astype(np.int64), cols, vals, 1024, 4, orc)  # windows of four columns
+    for order in ("tile", "window"):
+        _check(ip.astype(np.int64), cols, vals, 1024, 4096, orc, order)
+        _check(ip.astype(np.int64), cols, vals, 1024, 4, orc, order)  # windows of four columns
 
 
 def test_empty_matrix():
