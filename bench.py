@@ -49,6 +49,9 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
+# L2 request rate of MI355X: 8 XCDs x 16 L2 channels x one request per clock
+# at the 2.4 GHz peak engine clock
+L2_REQ_PEAK = 8 * 16 * 2.4e9
 METRIC = "SpMV/SpMM GB/s & %HBM peak; learned+batched vs plain at fixed rel err"
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
@@ -61,19 +64,24 @@ def spmv_bytes(n_rows, nnz, s_val=4, s_idx=4, s_ptr=4, s_x=4, s_y=4):
     return (n_rows + 1) * s_ptr + nnz * (s_idx + s_val + s_x) + n_rows * s_y
 
 
-def _traffic_from_profile(name="spmv_powerlaw_4m_pmc.txt"):
-    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE),
-    scaled by the copy-kernel calibration, from the committed summary."""
+def _profile_value(name, key):
+    """A `key = value` line of a committed rocprofv3 summary under profiles/."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
+        if line.startswith(key + " ") or line.startswith(key + "="):
             try:
                 return float(line.split("=", 1)[1].strip())
             except ValueError:
                 return None
     return None
+
+
+def _traffic_from_profile(name="spmv_powerlaw_4m_pmc.txt"):
+    """HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE),
+    scaled by the copy-kernel calibration, from the committed summary."""
+    return _profile_value(name, "traffic_bytes_per_launch")
 
 
 def cpu_baseline(ip, cols, vals, x, nbytes, budget_s=10.0):
@@ -260,6 +268,7 @@ def bench_spmm(args):
     stream = torch.cuda.current_stream(dev)
     us = _event_time(lambda: flagsparse_spmm_csr_opt(B=B, prepared=p), args.steps, stream)
     nbytes = (n + 1) * 4 + nnz * (4 + s_v + N * s_v) + n * N * s_v
+    compulsory = (n + 1) * 4 + nnz * (4 + s_v) + 2 * n * N * s_v
     gbs = nbytes / us / 1e3
     res = {"metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1,
            "steps": args.steps, "warmup": 3, "ms_per_step": round(us / 1e3, 5),
@@ -267,11 +276,18 @@ def bench_spmm(args):
            "data": "synthetic",
            "config": {"workload": "spmm_banded_1m", "n_rows": n, "nnz": nnz, "n_dense": N,
                       "gflops": round(2.0 * nnz * N / us / 1e3, 2),
-                      "compulsory_bytes": (n + 1) * 4 + nnz * (4 + s_v) + 2 * n * N * s_v,
+                      "compulsory_bytes": compulsory,
                       "schedule": repr(launch[0]), "col_tile": launch[1]},
-           "roofline": {"bound": "hbm", "kernel": "spmm_csr_kernel", "achieved": round(gbs, 2),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                        "traffic": None, "algorithmic_bytes_per_launch": nbytes,
+           # roofline over the compulsory bytes (A, B and C once each): the
+           # gather-counted rate above exceeds the HBM peak because B rows are
+           # re-read from the caches, so it bounds nothing
+           "roofline": {"bound": "hbm", "kernel": "spmm_csr_kernel",
+                        "achieved": round(compulsory / us / 1e3, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(compulsory / us / 1e3 / HBM_PEAK_GBS, 4),
+                        "traffic": (_traffic_from_profile("spmm_banded_1m_n64_pmc.txt")
+                                    if N == 64 and args.dtype == "f32" else None),
+                        "compulsory_bytes_per_launch": compulsory,
+                        "gather_counted_bytes_per_launch": nbytes,
                         "avg_launch_us": round(us, 3)}}
     if not args.no_vendor and vendor.available():
         C = torch.empty((n, N), dtype=B.dtype, device=dev)
@@ -881,6 +897,17 @@ def main(argv=None):
     }
     if kernels is not None:
         result["roofline"]["kernels"] = kernels
+    # the second bound of a gather-heavy SpMV: L2 requests per launch (TCC_REQ,
+    # rocprofv3, committed summary) over the launch time measured here
+    reqs = (_profile_value("spmv_powerlaw_4m_xwin_pmc.txt" if xw is not None
+                           else "spmv_powerlaw_4m_l2_counters.txt", "l2_requests_per_launch")
+            if args.dtype == "f32" else None)
+    if reqs:
+        rate = reqs / (kernel_us * 1e-6)
+        result["roofline_l2"] = {"bound": "l2_requests", "kernel": kname,
+                                 "achieved": round(rate / 1e9, 2), "peak": L2_REQ_PEAK / 1e9,
+                                 "unit": "G requests/s", "frac": round(rate / L2_REQ_PEAK, 4),
+                                 "requests_per_launch": int(reqs)}
     if not args.no_vendor:
         result["vendor_baseline"] = vendor_spmv(d_vals, d_cols, d_ip, d_x, n, nbytes, args.steps)
     modes = None
