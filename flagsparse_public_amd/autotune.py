@@ -26,13 +26,13 @@ from .rowsplit import Schedule
 
 # Version of each operation's variant list: a timing database or a model
 # recorded for another list is not used.
-VARIANTS_VERSION = {"spmv": 5, "spmm64": 4}
+VARIANTS_VERSION = {"spmv": 6, "spmm64": 4}
 
 # SpMV (fp32, native accumulation) variants.  0-5: entry tiles (fixed runs of
 # entries), 3-5 with the columns cut into L2-sized slices when x is larger
 # than one slice; 6-8: row tiles with chunked long rows; 9-15: one lane group
-# per row (S lanes), long rows chunked; 16-17: entry tiles over x gathered
-# from LDS-staged windows (128 KB / 160 KB of x per window).
+# per row (S lanes), long rows chunked; 16-17: entry tiles of 2048 / 4096
+# over x gathered from LDS-staged windows of 128 KB (window-major xg).
 SPMV_VARIANTS = (
     Schedule(tile_nnz=1024, tile_mode=1),
     Schedule(tile_nnz=2048, tile_mode=1),
@@ -51,7 +51,7 @@ SPMV_VARIANTS = (
     Schedule(group_lanes=32, short_max=512, chunk_nnz=1024, unroll=4),
     Schedule(group_lanes=64, short_max=4096, chunk_nnz=4096, unroll=8),
     Schedule(tile_nnz=2048, tile_mode=1, x_window=-1),
-    Schedule(tile_nnz=4096, tile_mode=1, x_window=40960),
+    Schedule(tile_nnz=4096, tile_mode=1, x_window=-1),
 )
 # The plain default-tile SpMV the learned selector is compared with: a whole
 # wavefront per row over fixed blocks of entries -- the structure of the
