@@ -62,12 +62,14 @@ SPMV_DEFAULT_INDEX = 15
 SPMV_FIXED_S8_INDEX = 12
 
 
-def spmv_heuristic_index(n_cols: int, n_rows: int | None = None, nnz: int | None = None) -> int:
+def spmv_heuristic_index(n_cols: int, n_rows: int | None = None, nnz: int | None = None,
+                         item: int = 4) -> int:
     """Variant chosen by the fixed rule of rowsplit.default_spmv_schedule for
-    fp32: entry tiles over 2 MB column slices once x exceeds 8 MB, else row
-    groups of width pow2ceil(mean row length / 2) (the nearest variant)."""
-    if n_cols * 4 > (8 << 20):
-        return 3
+    values of ``item`` bytes: entry tiles over column slices once x exceeds
+    8 MB (2 MB slices: variant 3 for fp32, 5 for fp64), else row groups of
+    width pow2ceil(mean row length / 2) (the nearest variant)."""
+    if n_cols * item > (8 << 20):
+        return 3 if item == 4 else 5
     if n_rows is None or nnz is None:
         return SPMV_FIXED_S8_INDEX
     from .rowsplit import rowgroup_spmv_schedule

@@ -6,8 +6,9 @@
   * "learned" (also what None means) runs the packaged selector for ``op`` on
     the device -- features from the analysis pass's statistics, MLP forward,
     argmin -- and falls back to the fixed rule when no model is installed, the
-    values are not fp32, or the matrix is larger than any the model was
-    trained on;
+    values are not fp32 (SpMV: fp32 or fp64; the SpMV model, trained on fp32
+    timings, was scored on held-out fp64 timings as well), or the matrix is
+    larger than any the model was trained on;
   * "exhaustive" is resolved after the prepared object exists
     (``exhaustive_spmv_schedule`` times every variant).
 Returns (schedule or None, how).
@@ -37,7 +38,7 @@ def choose_schedule(op, request, data, n_rows, n_cols, stats_dev):
         if op != "spmv":
             raise ValueError("exhaustive schedule search is offered for SpMV")
         return fallback, "exhaustive"
-    if data.dtype != torch.float32:
+    if data.dtype != torch.float32 and not (op == "spmv" and data.dtype == torch.float64):
         return fallback, "default"
     from .selector import load_default
     sel = load_default(op, data.device)

@@ -14,6 +14,13 @@ matrices the on-device selection is compared with exhaustive autotuning
 
 Writes DIR/{spmv,spmm64}_db.npz, DIR/{spmv,spmm64}_f32_mlp.json and
 DIR/eval.json; prints one progress line per 20 matrices.
+
+    python -m flagsparse_public_amd.train_selector --out DIR --value-dtype float64 --eval-only
+
+times every SpMV variant on the held-out matrices with fp64 values
+(DIR/spmv_f64_test_db.npz) and scores the packaged fp32-trained SpMV model
+on them (eval.json "spmv_f64_held_out"): the features depend only on the
+sparsity structure, so the question is whether the best variant does too.
 """
 
 from __future__ import annotations
@@ -46,10 +53,10 @@ def _log(msg):
 
 
 def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=100,
-            deadline=None):
-    """Time every variant of each operation in ``dbs`` on each matrix;
-    returns False when the deadline (time.time() value) stopped collection
-    early."""
+            deadline=None, value_dtype=torch.float32):
+    """Time every variant of each operation in ``dbs`` on each matrix (values
+    and x of ``value_dtype``); returns False when the deadline (time.time()
+    value) stopped collection early."""
     dense_cache = {}
     for k, seed in enumerate(seeds):
         if deadline is not None and time.time() > deadline:
@@ -58,13 +65,14 @@ def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=10
         family, n = synthetic.selector_matrix_spec(seed)
         n = min(n, max_n)
         indptr, cols, vals, params = synthetic.family_csr_torch(family, n, seed, device)
+        vals = vals.to(value_dtype)
         nnz = int(vals.numel())
         pv = prepare_spmv_csr(vals, cols, indptr, (n, n), schedule="default")
         stats = _lib.analyze_device(pv.csr_struct(), device)
         feats = _lib.features(stats, n, n, nnz, device).double().cpu().numpy()
         g = torch.Generator(device=device)
         g.manual_seed(seed + 1)
-        x = torch.rand(n, generator=g, device=device) * 2 - 1
+        x = (torch.rand(n, generator=g, device=device) * 2 - 1).to(value_dtype)
         best = []
         if "spmv" in dbs:
             tv = tune_spmv(pv, x, reps=reps)
@@ -148,7 +156,16 @@ def main(argv=None):
                     help="stop collecting after this many seconds (saved; resume later)")
     ap.add_argument("--ops", default="spmv,spmm64",
                     help="operations to collect and train (the others' files are kept)")
+    ap.add_argument("--value-dtype", choices=["float32", "float64"], default="float32")
+    ap.add_argument("--eval-only", action="store_true",
+                    help="with --value-dtype float64: time the held-out matrices in fp64 and "
+                         "score the packaged SpMV model on them")
     args = ap.parse_args(argv)
+    if args.value_dtype == "float64" or args.eval_only:
+        if not (args.value_dtype == "float64" and args.eval_only):
+            raise SystemExit("fp64 timings are collected to evaluate the packaged model: "
+                             "use --value-dtype float64 --eval-only")
+        return eval_f64(args)
     ops = [o for o in args.ops.split(",") if o]
     for o in ops:
         if o not in ("spmv", "spmm64"):
@@ -213,6 +230,47 @@ def main(argv=None):
     with open(eval_path, "w") as fh:
         json.dump(report, fh, indent=1)
     print(json.dumps(report), flush=True)
+    return 0
+
+
+def eval_f64(args):
+    """Held-out matrices with fp64 values: time every SpMV variant, score the
+    packaged fp32-trained model (and the fp64 fixed rule) on them."""
+    from .selector import load_default
+    os.makedirs(args.out, exist_ok=True)
+    device = torch.device("cuda", 0)
+    name = torch.cuda.get_device_name(device)
+    t0 = time.time()
+    path = os.path.join(args.out, "spmv_f64_test_db.npz")
+    db = TimingDB.load(path) if (args.resume and os.path.exists(path)) else \
+        TimingDB("spmv", len(SPMV_VARIANTS), name)
+    done = set(db.ids)
+    seeds = [s for seed0, count in ((TEST_SEED0, args.n_test), (LARGE_TEST_SEED0, args.n_large_test))
+             for s in range(seed0, seed0 + count) if s not in done]
+    deadline = t0 + args.time_budget if args.time_budget > 0 else None
+    complete = collect(seeds, device, {"spmv": db}, args.max_n, args.reps, t0, "test f64",
+                       save=lambda: db.save(path), deadline=deadline,
+                       value_dtype=torch.float64)
+    db.save(path)
+    if not complete:
+        _log(f"collected: {len(db)} fp64 held-out matrices")
+        return 0
+    sel = load_default("spmv", device)
+    if sel is None:
+        raise SystemExit("no packaged SpMV model to evaluate")
+    ev, _ = evaluate(sel, db, SPMV_DEFAULT_INDEX, device,
+                     lambda nc, nr, z: spmv_heuristic_index(nc, nr, z, item=8))
+    eval_path = os.path.join(args.out, "eval.json")
+    report = {}
+    if os.path.exists(eval_path):
+        with open(eval_path) as fh:
+            report = json.load(fh)
+    report["spmv_f64_held_out"] = ev
+    with open(eval_path, "w") as fh:
+        json.dump(report, fh, indent=1)
+    _log(f"fp64: regret {ev['geomean_regret']:.3f}, within 5% {ev['frac_within_5pct_of_exhaustive']:.3f}, "
+         f"vs fixed rule {ev.get('geomean_speedup_vs_heuristic', 0):.3f}")
+    print(json.dumps(ev), flush=True)
     return 0
 
 

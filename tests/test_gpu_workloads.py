@@ -178,6 +178,35 @@ def test_learned_spmv_on_held_out(orc, seed):
     assert row_rel_err(host64(y)[rows], y64, cond).max() <= 1e-5
 
 
+@pytest.mark.parametrize("seed", [7000, 7002, 7013, 9500])
+def test_learned_spmv_fp64_on_held_out(orc, seed):
+    """fp64 values: the fp32-trained SpMV model chooses from the same
+    structure features (recorded in the fp64 held-out database), and the
+    chosen kernel meets the fp64 bar on a sample of rows."""
+    db = TimingDB.load(os.path.join(DB_DIR, "spmv_f64_test_db.npz")) \
+        if os.path.exists(os.path.join(DB_DIR, "spmv_f64_test_db.npz")) else None
+    if db is None:
+        pytest.skip("no fp64 evaluation committed")
+    _, model = _load("spmv")
+    family, n = synthetic.selector_matrix_spec(seed)
+    indptr, cols, vals, _ = synthetic.family_csr_torch(family, n, seed, DEV)
+    vals = vals.double()
+    p = prepare_spmv_csr(vals, cols, indptr, (n, n))
+    assert p.schedule_record["chosen_by"].startswith(("learned:", "default (outside"))
+    _check_selection(orc, "spmv", seed, p, db, model)
+    g = torch.Generator(device=DEV)
+    g.manual_seed(seed + 1)
+    x = (torch.rand(n, generator=g, device=DEV) * 2 - 1).double()
+    y = flagsparse_spmv_csr(x=x, prepared=p, use_opt=True)
+    ip, ch, vh = indptr.cpu().numpy(), cols.cpu().numpy(), vals.cpu().numpy()
+    rows = _row_sample(ip, np.random.default_rng(seed))
+    sip, sc, sv, _ = _sub_csr(ip, ch, vh, rows)
+    xh = x.cpu().numpy()
+    y64 = orc.spmv(sip, sc, sv, xh)
+    cond = orc.spmv_abs(sip, sc, sv, xh)
+    assert row_rel_err(host64(y)[rows], y64, cond).max() <= 1e-12
+
+
 @pytest.mark.parametrize("seed", HELD_OUT[:8] + HELD_OUT[10:11])
 def test_learned_spmm64_on_held_out(orc, seed):
     db, model = _load("spmm64")

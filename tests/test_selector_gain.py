@@ -79,3 +79,31 @@ def test_selector_gain_on_held_out(orc, op, default_index):
         heur = np.asarray([spmv_heuristic_index(int(n), int(n), int(z)) for n, z in sizes])
         t_heu = times[rows, heur]
         assert float(np.exp(np.mean(np.log(t_heu / t_sel)))) >= 0.97
+
+
+def test_spmv_model_on_fp64_held_out(orc):
+    """The SpMV model is trained on fp32 timings and also chooses for fp64
+    matrices (its features depend only on the sparsity structure).  The
+    held-out matrices timed with fp64 values (spmv_f64_test_db.npz): its
+    choices, recomputed here, are those recorded on the device, close to
+    exhaustive autotuning and faster than the fp64 fixed rule."""
+    if not _have("spmv_f64_test_db.npz", "spmv_f32_mlp.json", "eval.json"):
+        pytest.skip("no fp64 evaluation committed")
+    db = TimingDB.load(os.path.join(DIR, "spmv_f64_test_db.npz"))
+    with open(os.path.join(DIR, "spmv_f32_mlp.json")) as fh:
+        model = json.load(fh)
+    with open(os.path.join(DIR, "eval.json")) as fh:
+        ev = json.load(fh)["spmv_f64_held_out"]
+    _, _, feats, times, sizes = db.arrays()
+    choice, _ = _choices(orc, model, feats)
+    rows = np.arange(times.shape[0])
+    t_sel = times[rows, choice]
+    t_best = times.min(axis=1)
+    regret = float(np.exp(np.mean(np.log(t_sel / t_best))))
+    assert abs(regret - ev["geomean_regret"]) <= 2e-3 * ev["geomean_regret"]
+    assert regret < 1.06
+    heur = np.asarray([spmv_heuristic_index(int(n), int(n), int(z), item=8) for n, z in sizes])
+    t_heu = times[rows, heur]
+    gain = float(np.exp(np.mean(np.log(t_heu / t_sel))))
+    assert abs(gain - ev["geomean_speedup_vs_heuristic"]) <= 2e-3 * gain
+    assert gain > 1.1
