@@ -79,3 +79,25 @@ def test_noalloc_equals_fs_spmv_csr(orc, name, dtype, accumulate):
         err = row_rel_err(host64(got), want, 0.75 * cond + 1.5 * np.abs(y0))
         assert err.max() <= tol
 
+
+
+@pytest.mark.parametrize("order", ["tile", "window"])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_noalloc_x_window_plans(order, dtype):
+    """The gathered-x form (x-window second pass, both xg orders): the carries
+    go to the scratch array, the results equal fs_spmv_csr's bit for bit."""
+    from flagsparse_public_amd.xwin import XWinCsr
+    for n, seed in ((150_000, 21), (20_000, 22)):  # second call smaller
+        ip, cols, vals, x, y0 = _problem(n, seed, dtype)
+        print(f"[noalloc] x-window {order} {dtype} n={n} nnz={vals.size}", flush=True)
+        d_ip, d_cols, d_vals, d_x = to_dev(ip), to_dev(cols), to_dev(vals), to_dev(x)
+        xw = XWinCsr(d_vals, d_cols, d_ip, n, n, tile_nnz=2048, window=4096, order=order)
+        _lib.xwin_gather(xw.layout, d_x, xw.xg, DEV)
+        st = xw.plan(_lib.FS_ACC_NATIVE).struct(1)
+        ref = to_dev(y0)
+        _lib.spmv(xw.csr, xw.xg, ref, 0.5, 2.0, st, DEV)
+        got = to_dev(y0)
+        rc = _noalloc(xw.csr, xw.xg, got, 0.5, 2.0, st)
+        assert rc == _lib.FS_OK, _lib.load().fs_last_error()
+        torch.cuda.synchronize()
+        assert torch.equal(got, ref), (order, n)
