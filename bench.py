@@ -415,6 +415,25 @@ def _rel_err(got, ref, cond):
     return float((d / np.where(cond > 0, cond, 1.0)).max()) if d.size else 0.0
 
 
+def _err_report(got, ref, cond, value_dtype):
+    """Per-row error of a result against the fp64 oracle, three ways: over
+    the condition sum Σ|a x| (the bar the tests use), over |y64| (rows with
+    y64 != 0; U(-1,1) data gives rows whose y is near zero, hence the median
+    beside the max), and the reference's allclose error ratio max |d| / (atol
+    + rtol |y64|) with its tolerances for the dtype (<= 1 passes allclose)."""
+    import numpy as np
+    from flagsparse_public_amd.sparse_operations._common import _tolerance_for_dtype
+    d = np.abs(np.asarray(got, np.float64) - ref)
+    atol, rtol = _tolerance_for_dtype(value_dtype)
+    nz = ref != 0
+    vs_y = d[nz] / np.abs(ref[nz]) if nz.any() else np.zeros(1)
+    return {"max_rel_err_vs_cond": _rel_err(got, ref, cond),
+            "median_rel_err_vs_abs_y": float(np.median(vs_y)),
+            "max_rel_err_vs_abs_y": float(vs_y.max()),
+            "allclose_error_ratio": float((d / (atol + rtol * np.abs(ref))).max()) if d.size else 0.0,
+            "allclose_tolerance": {"atol": atol, "rtol": rtol}}
+
+
 def sub_spmv_f64(steps):
     """The 4M-row power-law SpMV in fp64 (library schedule, native path)."""
     import numpy as np
@@ -938,9 +957,14 @@ def main(argv=None):
             yh = yd.double().cpu().numpy()
             return float((np.abs(yh - y64) / np.where(cond > 0, cond, 1.0)).max())
         result["config"]["max_rel_err_vs_fp64"] = rel_err(y)
+        result["config"]["error_vs_fp64"] = _err_report(y.double().cpu().numpy(), y64, cond,
+                                                        d_vals.dtype)
         if modes is not None:
             for m in modes.values():
-                m["max_rel_err_vs_fp64"] = rel_err(m.pop("_y"))
+                ym = m.pop("_y")
+                m["max_rel_err_vs_fp64"] = rel_err(ym)
+                m["error_vs_fp64"] = _err_report(ym.double().cpu().numpy(), y64, cond,
+                                                 torch.float32)
     if modes is not None:
         for m in modes.values():
             m.pop("_y", None)
