@@ -205,3 +205,56 @@ def test_dtype_codes_cover_reference_tables():
         _lib.value_code(d)
     with pytest.raises(TypeError):
         _lib.value_code(torch.int8)
+
+
+def _entry_plan(**kw):
+    base = dict(short_max=0, chunk_nnz=64, tile_nnz=2048, n_tiles=1, tile_start=96,
+                tile_mode=_lib.FS_TILE_ENTRIES, tile_entry=112, partials=8, partials_bytes=1024)
+    base.update(kw)
+    return _plan(**base)
+
+
+@pytest.mark.parametrize("plan,msg", [
+    # a gathered-x run table without x_perm, outside and inside entry tiles
+    (_plan(xg_run_off=8), "run table is offered with x_perm only"),
+    (_entry_plan(xg_tile_run=8, xg_batch_run=8, xg_batch_mask=8, xg_run_off=8, xg_n_runs=4),
+     "run table is offered with x_perm only"),
+    # window-major xg with a missing piece of its run table
+    (_entry_plan(x_perm=8, xg_run_off=8, xg_batch_run=8, xg_batch_mask=8, xg_n_runs=4),
+     "window-major xg needs"),
+    (_entry_plan(x_perm=8, xg_run_off=8, xg_tile_run=8, xg_batch_run=8, xg_batch_mask=8,
+                 xg_n_runs=0), "window-major xg needs"),
+])
+def test_spmv_rejects_bad_xg_run_tables(plan, msg):
+    """Gathered-x plans whose window-major run table is incomplete are refused
+    before launch (the pointers are never dereferenced)."""
+    rc = _spmv(_csr(), plan)
+    assert rc == _lib.FS_ERR_VALUE
+    assert msg in _lib.load().fs_last_error().decode()
+
+
+def _xwin(**kw):
+    base = dict(value_type=_lib.FS_F32, window=1024, n_cols=4096, nnz=100, n_runs=10, n_parts=2,
+                wcol=16, batch_run=32, batch_mask=48, run_off=64, part_window=80,
+                part_entry=96, window_entry=112, order=_lib.FS_XG_WINDOW_MAJOR, reserved=0)
+    base.update(kw)
+    return _lib.FsXwin(**base)
+
+
+@pytest.mark.parametrize("layout,xg,msg", [
+    (_xwin(order=7), 256, "order must be"),
+    (_xwin(value_type=_lib.FS_C64), 256, "fp32 / fp64"),
+    (_xwin(window=1000 + 2), 256, "multiple of 4"),
+    (_xwin(nnz=2**31), 256, "2^31 - 1"),
+    (_xwin(wcol=18), 256, "16-byte aligned"),       # window-major: 16-byte wcol and xg
+    (_xwin(), 260, "16-byte aligned"),
+    (_xwin(order=_lib.FS_XG_TILE_MAJOR, run_off=None), 256, "null array"),
+    (_xwin(order=_lib.FS_XG_TILE_MAJOR, n_runs=0), 256, "without runs"),
+])
+def test_xwin_gather_rejects_bad_layouts(layout, xg, msg):
+    """fs_xwin_gather's argument checks, all before any launch; a
+    window-major layout needs no run arrays (they may be null)."""
+    lib = _lib.load()
+    rc = lib.fs_xwin_gather(ctypes.byref(layout), 512, xg, None)
+    assert rc in (_lib.FS_ERR_VALUE, _lib.FS_ERR_TYPE, _lib.FS_ERR_UNSUPPORTED)
+    assert msg in lib.fs_last_error().decode()
