@@ -13,7 +13,7 @@ import pytest
 import torch
 
 from flagsparse_public_amd import _lib, flagsparse_spmv_csr, prepare_spmv_csr, synthetic
-from flagsparse_public_amd.rowsplit import Schedule
+from flagsparse_public_amd.rowsplit import RowSplitPlan, Schedule
 from flagsparse_public_amd.xwin import XWinCsr
 from tests._util import host64, row_rel_err, to_dev, ulp_distance_f32
 
@@ -144,3 +144,51 @@ def test_xwin_powerlaw_4m_full_size(orc):
         y = flagsparse_spmv_csr(x=dx, prepared=xw, use_opt=use_opt)
         assert torch.equal(y, flagsparse_spmv_csr(x=dx, prepared=ref, use_opt=use_opt))
         _check(y, y64, cond, torch.float32, use_opt)
+
+
+@pytest.mark.parametrize("case", range(16))
+def test_xwin_random_shapes(case):
+    """Seeded random shapes: sizes from a few rows to 60k, densities from
+    mostly-empty to ~60 entries per row with runs of one-entry rows (row-cap
+    tile cuts off 64-entry batches), windows from 4 to 65,532 columns, tile
+    sizes 1024 / 2048 / 4096, both xg orders, fp32 / fp64, int32 / int64
+    indices: bit-identical to the one-pass entry tiles."""
+    rng = np.random.default_rng(1000 + case)
+    dtype = (torch.float32, torch.float64)[case % 2]
+    npd = np.float32 if dtype == torch.float32 else np.float64
+    n = int(np.exp(rng.uniform(np.log(5), np.log(60_000))))
+    n_cols = int(np.exp(rng.uniform(np.log(4), np.log(300_000))))
+    lens = rng.poisson(rng.uniform(0.05, 60), n)
+    if case % 3 == 0:
+        lens[: n // 2] = 1  # one-entry rows: tiles cut every 1024 rows
+    lens = np.minimum(lens, n_cols)
+    ip = np.zeros(n + 1, np.int64)
+    np.cumsum(lens, out=ip[1:])
+    nnz = int(ip[-1])
+    cols = rng.integers(0, n_cols, nnz)
+    vals = rng.uniform(-1, 1, nnz).astype(npd)
+    idx_dtype = (np.int32, np.int64)[(case // 2) % 2]
+    T = (1024, 2048, 4096)[case % 3] if dtype == torch.float32 else (1024, 2048)[case % 2]
+    max_w = 32768 if dtype == torch.float32 else 16384
+    W = int(rng.choice([4, 64, 1000, 4096, max_w]))
+    W -= W % 4
+    order = ("tile", "window")[(case // 4) % 2]
+    print(f"[xwin-rand] case {case}: n={n} n_cols={n_cols} nnz={nnz} {dtype} T={T} W={W} "
+          f"{order}", flush=True)
+    args = [to_dev(vals), to_dev(cols.astype(idx_dtype)), to_dev(ip.astype(idx_dtype))]
+    x = to_dev(rng.uniform(-1, 1, n_cols).astype(npd))
+    # the one-pass entry tiles launched directly with native accumulation, on
+    # int32 copies of the indices (an int64-index prepared matrix takes the
+    # default path even with use_opt, as in the reference -- supports_opt needs
+    # int32 columns -- and one-pass tiles of 1024 / 4096 are built for int32
+    # columns only; the index type does not enter the arithmetic)
+    a32 = [args[0], args[1].to(torch.int32), args[2].to(torch.int32)]
+    plan = RowSplitPlan(a32[2], n, dtype, Schedule(tile_nnz=T, tile_mode=1), _lib.FS_ACC_NATIVE)
+    y_ref = torch.empty(n, dtype=dtype, device=x.device)
+    _lib.spmv(_lib.make_csr(*a32, n, n_cols), x, y_ref, 1.0, 0.0, plan.struct(1), x.device)
+    if nnz == 0:
+        return
+    xw = XWinCsr(*args, n, n_cols, tile_nnz=T, window=W, order=order)
+    y = torch.empty_like(y_ref)
+    xw.run(x, y, 1.0, 0.0, _lib.FS_ACC_NATIVE, x.device)
+    assert torch.equal(y, y_ref)
