@@ -139,11 +139,15 @@ def time_launches(fn, reps: int = 7, warmup: int = 2, inner: int = 1) -> float:
 
 
 def tune_spmv(prepared, x, variants=SPMV_VARIANTS, reps=7, warmup=2, inner=1):
-    """Median us of every SpMV variant (native accumulation) on one matrix."""
+    """Median us of every SpMV variant (native accumulation) on one matrix;
+    each variant runs as what is built for the matrix's dtypes
+    (rowsplit.supported_spmv_schedule)."""
+    from .rowsplit import supported_spmv_schedule
     from .sparse_operations.spmv_csr import _hip_spmv_csr_impl_opt_prepared
     y = torch.empty(prepared.n_rows, dtype=prepared.data.dtype, device=prepared.data.device)
     times = []
-    for sch in variants:
+    dt, it = prepared.data.dtype, prepared.kernel_indices.dtype
+    for sch in (supported_spmv_schedule(v, dt, it) for v in variants):
         times.append(time_launches(
             lambda s=sch: _hip_spmv_csr_impl_opt_prepared(prepared, x, out=y, schedule=s),
             reps, warmup, inner))

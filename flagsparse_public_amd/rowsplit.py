@@ -71,6 +71,20 @@ def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None, n_cols: int |
     return rowgroup_spmv_schedule(n_rows, nnz)
 
 
+def supported_spmv_schedule(sch: Schedule, value_dtype, index_dtype) -> Schedule:
+    """The schedule a library-chosen variant runs as for these dtypes: entry
+    tiles of 1024 / 4096 entries are built for fp32 / fp64 values with int32
+    columns only (others: tiles of 2048), and the x-window layout (whose second
+    pass indexes through 16-bit positions, any column type) for fp32 / fp64
+    values only."""
+    if sch.tile_mode != 1 or sch.tile_nnz in (0, 2048):
+        return sch
+    fp = value_dtype in (torch.float32, torch.float64)
+    if fp and (sch.x_window != 0 or index_dtype == torch.int32):
+        return sch
+    return dataclasses.replace(sch, tile_nnz=2048)
+
+
 def row_tile_spmv_schedule(n_rows: int, nnz: int, value_dtype=None) -> Schedule:
     """Row tiles of ~2048 entries for rows up to 2048 entries (1024 for
     complex values, whose fp64 row sums take 16 bytes of LDS each), longer

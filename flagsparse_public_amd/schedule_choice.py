@@ -58,5 +58,7 @@ def exhaustive_spmv_schedule(prepared, reps=5):
     g.manual_seed(0)
     x = (torch.rand(prepared.n_cols, generator=g, device=prepared.data.device) * 2 - 1).to(
         prepared.data.dtype)
+    from .rowsplit import supported_spmv_schedule
     times = tune_spmv(prepared, x, reps=reps)
-    return best_schedule(times, SPMV_VARIANTS)[1]
+    return supported_spmv_schedule(best_schedule(times, SPMV_VARIANTS)[1], prepared.data.dtype,
+                                   prepared.kernel_indices.dtype)

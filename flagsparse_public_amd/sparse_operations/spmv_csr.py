@@ -23,7 +23,7 @@ import time
 import torch
 
 from .. import _lib
-from ..rowsplit import RowSplitPlan, Schedule, default_spmv_schedule
+from ..rowsplit import RowSplitPlan, Schedule, default_spmv_schedule, supported_spmv_schedule
 from ..schedule_choice import choose_schedule, exhaustive_spmv_schedule
 from ..slicing import SLICED_DTYPES, SlicedCsr
 from ..xwin import XWIN_DTYPES, XWinCsr
@@ -362,6 +362,8 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
     else:
         max_segments_use = max_segments
     sch, how = choose_schedule("spmv", schedule, data, n_rows, n_cols, stats_out[0][0])
+    if how != "given":  # a library choice runs as what is built for these dtypes
+        sch = supported_spmv_schedule(sch, data.dtype, kernel_indices.dtype)
     prepared = PreparedCsrSpmv(
         data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
         n_rows=n_rows, n_cols=n_cols, block_nnz=block_nnz_use, max_segments=max_segments_use,

@@ -455,3 +455,23 @@ def test_spmm_candidate_buckets_match_length_ranges(dtype):
     assert np.all(seen == 1)
     kinds = {sp["label"]: sp["kind"] for sp in specs}
     assert kinds["short_16"] == ("segmented" if dtype == torch.float64 else "micro")
+
+
+def test_supported_spmv_schedule():
+    """Library-chosen entry tiles of 1024 / 4096 run as 2048 where only 2048
+    is built (int64 columns, values other than fp32 / fp64); x-window
+    schedules keep their tiles for fp32 / fp64 with any column type."""
+    import torch
+    from flagsparse_public_amd.rowsplit import Schedule, supported_spmv_schedule
+    t4 = Schedule(tile_nnz=4096, tile_mode=1)
+    xw = Schedule(tile_nnz=4096, tile_mode=1, x_window=-1)
+    rows = Schedule(group_lanes=16)
+    assert supported_spmv_schedule(t4, torch.float32, torch.int32) == t4
+    assert supported_spmv_schedule(t4, torch.float64, torch.int64).tile_nnz == 2048
+    assert supported_spmv_schedule(t4, torch.float16, torch.int32).tile_nnz == 2048
+    assert supported_spmv_schedule(xw, torch.float32, torch.int64) == xw
+    assert supported_spmv_schedule(xw, torch.complex64, torch.int32).tile_nnz == 2048
+    assert supported_spmv_schedule(rows, torch.float16, torch.int64) == rows
+    sliced = Schedule(tile_nnz=1024, tile_mode=1, x_slice_cols=1 << 18)
+    out = supported_spmv_schedule(sliced, torch.float32, torch.int64)
+    assert out.tile_nnz == 2048 and out.x_slice_cols == 1 << 18
