@@ -579,6 +579,7 @@ def sub_selection(d_spmv, spmv_prepared, steps):
     from flagsparse_public_amd.autotune import (SPMM_DEFAULT_INDEX, SPMM_VARIANTS,
                                                 SPMV_DEFAULT_INDEX, SPMV_VARIANTS,
                                                 spmv_heuristic_index, tune_spmm, tune_spmv)
+    from flagsparse_public_amd.rowsplit import columns_local
     dev = torch.device("cuda", 0)
     out = {}
     vals, cols, ip, x = d_spmv
@@ -586,8 +587,10 @@ def sub_selection(d_spmv, spmv_prepared, steps):
     how = spmv_prepared.schedule_record["chosen_by"]
     li = int(how.split(":")[1]) if how.startswith("learned:") else None
     n = spmv_prepared.n_rows
+    local = columns_local(ip, cols, n, n, vals.element_size())
     out["spmv_powerlaw_4m"] = _selection_record(t, li, SPMV_DEFAULT_INDEX,
-                                                spmv_heuristic_index(n, n, int(vals.numel())),
+                                                spmv_heuristic_index(n, n, int(vals.numel()),
+                                                                     local=local),
                                                 how)
     ipb, colsb, valsb = synthetic.spmm_banded_1m()
     nb = ipb.size - 1
@@ -698,6 +701,7 @@ def _selection_record(times, learned, default, rule, how):
     best = int(times.argmin())
     rec = {"variants_timed": int(times.size), "exhaustive_best": best,
            "exhaustive_us": round(float(times[best]), 2),
+           "variant_us": [round(float(t), 2) for t in times],
            "default_tile_us": round(float(times[default]), 2), "chosen_by": how}
     if learned is not None:
         rec["learned"] = learned

@@ -63,12 +63,17 @@ SPMV_FIXED_S8_INDEX = 12
 
 
 def spmv_heuristic_index(n_cols: int, n_rows: int | None = None, nnz: int | None = None,
-                         item: int = 4) -> int:
+                         item: int = 4, local: bool | None = None) -> int:
     """Variant chosen by the fixed rule of rowsplit.default_spmv_schedule for
-    values of ``item`` bytes: entry tiles over column slices once x exceeds
-    8 MB (2 MB slices: variant 3 for fp32, 5 for fp64), else row groups of
-    width pow2ceil(mean row length / 2) (the nearest variant)."""
+    fp32 (``item`` 4) or fp64 (8) values.  x larger than 8 MB: with the
+    matrix's column locality known (``local``, rowsplit.columns_local), plain
+    entry tiles of 2048 (variant 1) for local columns and the x-window tiles
+    of 4096 (17) otherwise; without it, entry tiles over 2 MB column slices
+    (3 for fp32, 5 for fp64).  Else row groups of width pow2ceil(mean row
+    length / 2) (the nearest variant)."""
     if n_cols * item > (8 << 20):
+        if local is not None:
+            return 1 if local else 17
         return 3 if item == 4 else 5
     if n_rows is None or nnz is None:
         return SPMV_FIXED_S8_INDEX

@@ -108,7 +108,8 @@ def prepare_spmv_csr_split(data, indices, indptr, shape, value_format="bf16x2", 
     if schedule is None:
         # the layout the plain fp32 SpMV of this matrix would use (the packaged
         # selector on the device, else the fixed rule)
-        sch, _ = choose_schedule("spmv", None, data, n_rows, n_cols, stats_out[0][0])
+        sch, _ = choose_schedule("spmv", None, data, n_rows, n_cols, stats_out[0][0],
+                                 kernel_indptr, kernel_indices)
     else:
         sch = schedule
     return PreparedCsrSpmvSplit(kernel_indices, kernel_indptr, (n_rows, n_cols), value_format,

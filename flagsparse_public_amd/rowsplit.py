@@ -57,17 +57,62 @@ L2_SLICE_BYTES = 2 << 20    # x columns per XCD slice: half of one XCD's 4 MB L2
 SLICE_MIN_X_BYTES = 8 << 20  # slice only when x is well beyond what L2 + MALL serve well
 
 
-def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None, n_cols: int | None = None
-                          ) -> Schedule:
-    """Fixed rule used when no selector is supplied.  x larger than 8 MB
-    (fp32 / fp64 / complex values): entry tiles of 2048 over the column-sliced
-    layout (2 MB of x per slice) -- random x gathers are then served from each
-    XCD's L2.  Otherwise one lane group per row, its width from the mean row
-    length (rowgroup_spmv_schedule)."""
+LOCALITY_SAMPLE_ROWS = 4096  # rows sampled by far_column_fraction
+LOCAL_X_BYTES = 128 << 10  # a row is local when its first column is this close to its diagonal
+
+
+def far_column_fraction(indptr: torch.Tensor, indices: torch.Tensor, n_rows: int, n_cols: int,
+                        far_cols: int) -> float | None:
+    """Fraction of up to LOCALITY_SAMPLE_ROWS evenly spaced nonempty rows
+    whose first column lies more than far_cols columns from the row's
+    diagonal position (row * n_cols // n_rows); None when no sampled row has
+    entries.  One small device pass and one host read."""
+    if n_rows <= 0 or indices.numel() == 0:
+        return None
+    dev = indptr.device
+    step = max(1, n_rows // LOCALITY_SAMPLE_ROWS)
+    rows = torch.arange(0, n_rows, step, device=dev, dtype=torch.int64)
+    start = indptr[rows].to(torch.int64)
+    nonempty = indptr[rows + 1].to(torch.int64) > start
+    first = indices[torch.clamp(start, max=indices.numel() - 1)].to(torch.int64)
+    diag = rows * n_cols // n_rows
+    far = ((first - diag).abs() > far_cols) & nonempty
+    counts = torch.stack([far.sum(), nonempty.sum()]).cpu().tolist()
+    return counts[0] / counts[1] if counts[1] else None
+
+
+def columns_local(indptr: torch.Tensor, indices: torch.Tensor, n_rows: int, n_cols: int,
+                  item: int) -> bool | None:
+    """Whether most sampled rows start within LOCAL_X_BYTES of x (values of
+    ``item`` bytes) of their diagonal: banded, block-diagonal and
+    bandwidth-reduced matrices are; random column patterns (uniform,
+    power-law, R-MAT) are not.  None for a matrix without entries."""
+    far = far_column_fraction(indptr, indices, n_rows, n_cols, max(1, LOCAL_X_BYTES // item))
+    return None if far is None else far < 0.5
+
+
+def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None, n_cols: int | None = None,
+                          indptr: torch.Tensor | None = None,
+                          indices: torch.Tensor | None = None) -> Schedule:
+    """Fixed rule used when no selector applies.  x larger than 8 MB: when
+    the matrix's row pointers and columns are given, a sample of rows
+    decides (columns_local) between plain entry tiles of 2048 (local
+    columns: the x gathers hit L2) and, for random columns, the x-window
+    layout with tiles of 4096 (fp32 / fp64 values) or entry tiles of 2048
+    over the column-sliced layout (2 MB of x per slice; other values, or
+    without the matrix's arrays).  Otherwise one lane group per row, its
+    width from the mean row length (rowgroup_spmv_schedule)."""
     if value_dtype is not None and n_cols is not None:
         item = torch.empty(0, dtype=value_dtype).element_size()
         if n_cols * item > SLICE_MIN_X_BYTES:
-            return Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=L2_SLICE_BYTES // item)
+            sliced = Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=L2_SLICE_BYTES // item)
+            if indptr is None or indices is None:
+                return sliced
+            if columns_local(indptr, indices, n_rows, n_cols, item):
+                return Schedule(tile_nnz=2048, tile_mode=1)
+            if value_dtype in (torch.float32, torch.float64):
+                return Schedule(tile_nnz=4096, tile_mode=1, x_window=-1)
+            return sliced
     return rowgroup_spmv_schedule(n_rows, nnz)
 
 

@@ -23,30 +23,35 @@ from .rowsplit import Schedule, default_spmm_schedule, default_spmv_schedule
 _MODES = ("learned", "default", "exhaustive")
 
 
-def choose_schedule(op, request, data, n_rows, n_cols, stats_dev):
+def choose_schedule(op, request, data, n_rows, n_cols, stats_dev, indptr=None, indices=None):
+    """``indptr`` / ``indices``: the matrix's row pointers and columns, which
+    the SpMV fixed rule samples for column locality when x is large."""
     nnz = data.numel()
     if isinstance(request, Schedule):
         return request, "given"
     mode = "learned" if request is None else str(request)
     if mode not in _MODES:
         raise ValueError(f"schedule must be a Schedule or one of {_MODES}")
-    fallback = (default_spmv_schedule(n_rows, nnz, data.dtype, n_cols) if op == "spmv"
-                else default_spmm_schedule(n_rows, nnz, 64))
+
+    def fallback():
+        if op == "spmv":
+            return default_spmv_schedule(n_rows, nnz, data.dtype, n_cols, indptr, indices)
+        return default_spmm_schedule(n_rows, nnz, 64)
     if mode == "default" or n_rows == 0:
-        return fallback, "default"
+        return fallback(), "default"
     if mode == "exhaustive":
         if op != "spmv":
             raise ValueError("exhaustive schedule search is offered for SpMV")
-        return fallback, "exhaustive"
+        return fallback(), "exhaustive"
     if data.dtype != torch.float32 and not (op == "spmv" and data.dtype == torch.float64):
-        return fallback, "default"
+        return fallback(), "default"
     from .selector import load_default
     sel = load_default(op, data.device)
     if sel is None:
-        return fallback, "default"
+        return fallback(), "default"
     if not sel.covers(n_rows, n_cols, nnz):
         # outside the sizes the network was trained on: the fixed rule
-        return fallback, "default (outside the selector's training range)"
+        return fallback(), "default (outside the selector's training range)"
     sch, idx = sel.schedule_for(stats_dev, n_rows, n_cols, nnz)
     return sch, f"learned:{idx}"
 

@@ -361,7 +361,8 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
             max_segments_use = max((max_row_nnz + block_nnz_use - 1) // block_nnz_use, 1)
     else:
         max_segments_use = max_segments
-    sch, how = choose_schedule("spmv", schedule, data, n_rows, n_cols, stats_out[0][0])
+    sch, how = choose_schedule("spmv", schedule, data, n_rows, n_cols, stats_out[0][0],
+                               kernel_indptr, kernel_indices)
     if how != "given":  # a library choice runs as what is built for these dtypes
         sch = supported_spmv_schedule(sch, data.dtype, kernel_indices.dtype)
     prepared = PreparedCsrSpmv(

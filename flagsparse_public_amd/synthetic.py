@@ -214,6 +214,9 @@ def spmm_banded_1m(seed=3, dtype=np.float32, n=1_048_576):
 # stored) except where the family fixes them; rows are sorted by column.
 
 SELECTOR_FAMILIES = ("poisson", "powerlaw", "banded", "blockdiag", "rmat")
+# Families whose rows' columns lie within a few dozen of the diagonal
+# (rowsplit.columns_local is true for them); the others draw columns at random.
+LOCAL_FAMILIES = ("banded", "blockdiag")
 
 
 def _torch_rows_to_csr(torch, rows_len, cols, n, vals_gen, device):

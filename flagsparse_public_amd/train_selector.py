@@ -15,6 +15,11 @@ matrices the on-device selection is compared with exhaustive autotuning
 Writes DIR/{spmv,spmm64}_db.npz, DIR/{spmv,spmm64}_f32_mlp.json and
 DIR/eval.json; prints one progress line per 20 matrices.
 
+    python -m flagsparse_public_amd.train_selector --out DIR --eval-only
+
+scores the packaged models on the held-out timings in DIR again (no
+collection, no training) and rewrites their reports in DIR/eval.json.
+
     python -m flagsparse_public_amd.train_selector --out DIR --value-dtype float64 --eval-only
 
 times every SpMV variant on the held-out matrices with fp64 values
@@ -99,7 +104,8 @@ def collect(seeds, device, dbs, max_n, reps, t0, label, save=None, save_every=10
 def evaluate(sel: Selector, db: TimingDB, default_index: int, device, heuristic=None):
     """Regret against exhaustive autotuning and speed-up over the plain
     default-tile variant (and over the fixed heuristic rule when given: a
-    function n_cols -> variant index)."""
+    function (n_cols, n_rows, nnz, local) -> variant index, ``local`` the
+    family's column locality, synthetic.LOCAL_FAMILIES)."""
     _, fams, feats, times, sizes = db.arrays()
     choices = []
     for f in feats:
@@ -123,7 +129,8 @@ def evaluate(sel: Selector, db: TimingDB, default_index: int, device, heuristic=
         "per_family": {},
     }
     if heuristic is not None:
-        hidx = np.asarray([heuristic(int(n), int(n), int(z)) for n, z in sizes])
+        hidx = np.asarray([heuristic(int(n), int(n), int(z), str(f) in synthetic.LOCAL_FAMILIES)
+                           for (n, z), f in zip(sizes, fams)])
         t_heu = times[rows, hidx]
         out["geomean_speedup_vs_heuristic"] = float(np.exp(np.mean(np.log(t_heu / t_sel))))
         out["heuristic_geomean_regret"] = float(np.exp(np.mean(np.log(t_heu / t_best))))
@@ -158,14 +165,14 @@ def main(argv=None):
                     help="operations to collect and train (the others' files are kept)")
     ap.add_argument("--value-dtype", choices=["float32", "float64"], default="float32")
     ap.add_argument("--eval-only", action="store_true",
-                    help="with --value-dtype float64: time the held-out matrices in fp64 and "
-                         "score the packaged SpMV model on them")
+                    help="score the packaged models without training: on the held-out timings "
+                         "in DIR (float32), or on held-out matrices timed in fp64 (float64)")
     args = ap.parse_args(argv)
-    if args.value_dtype == "float64" or args.eval_only:
-        if not (args.value_dtype == "float64" and args.eval_only):
-            raise SystemExit("fp64 timings are collected to evaluate the packaged model: "
-                             "use --value-dtype float64 --eval-only")
-        return eval_f64(args)
+    if args.value_dtype == "float64" and not args.eval_only:
+        raise SystemExit("fp64 timings are collected to evaluate the packaged model: "
+                         "use --value-dtype float64 --eval-only")
+    if args.eval_only:
+        return eval_f64(args) if args.value_dtype == "float64" else eval_f32(args)
     ops = [o for o in args.ops.split(",") if o]
     for o in ops:
         if o not in ("spmv", "spmm64"):
@@ -218,7 +225,7 @@ def main(argv=None):
         _, _, _, _, sizes = dbs["train"][op].arrays()
         sel.meta.update({"device": name, "default_index": default_index,
                          "n_max": int(sizes[:, 0].max()), "nnz_max": int(sizes[:, 1].max())})
-        heur = spmv_heuristic_index if op == "spmv" else None
+        heur = _heuristic_f32 if op == "spmv" else None
         ev, _ = evaluate(sel, dbs["test"][op], default_index, device, heur)
         tr, _ = evaluate(sel, dbs["train"][op], default_index, device, heur)
         report[op] = {"held_out": ev, "train": {k: tr[k] for k in
@@ -230,6 +237,48 @@ def main(argv=None):
     with open(eval_path, "w") as fh:
         json.dump(report, fh, indent=1)
     print(json.dumps(report), flush=True)
+    return 0
+
+
+def _heuristic_f32(n_cols, n_rows, nnz, local):
+    return spmv_heuristic_index(n_cols, n_rows, nnz, local=local)
+
+
+def eval_f32(args):
+    """Score the packaged models on the held-out timings already in DIR
+    (DIR/{op}_test_db.npz) without collecting or training: rewrites each
+    operation's "held_out" report in DIR/eval.json and the model's copy of
+    it."""
+    from .selector import load_default
+    device = torch.device("cuda", 0)
+    eval_path = os.path.join(args.out, "eval.json")
+    report = {}
+    if os.path.exists(eval_path):
+        with open(eval_path) as fh:
+            report = json.load(fh)
+    defaults = {"spmv": SPMV_DEFAULT_INDEX, "spmm64": SPMM_DEFAULT_INDEX}
+    for op in [o for o in args.ops.split(",") if o]:
+        path = os.path.join(args.out, f"{op}_test_db.npz")
+        if op not in defaults or not os.path.exists(path):
+            raise SystemExit(f"no held-out timings for {op} in {args.out}")
+        sel = load_default(op, device)
+        if sel is None:
+            raise SystemExit(f"no packaged {op} model to evaluate")
+        heur = _heuristic_f32 if op == "spmv" else None
+        ev, _ = evaluate(sel, TimingDB.load(path), defaults[op], device, heur)
+        report.setdefault(op, {})["held_out"] = ev
+        mpath = os.path.join(args.out, f"{op}_f32_mlp.json")
+        if os.path.exists(mpath):  # the model's copy of its report; parameters untouched
+            with open(mpath) as fh:
+                d = json.load(fh)
+            d.setdefault("meta", {})["held_out"] = ev
+            with open(mpath, "w") as fh:
+                json.dump(d, fh)
+        _log(f"{op}: held-out regret {ev['geomean_regret']:.3f}, speed-up vs default "
+             f"{ev['geomean_speedup_vs_default']:.3f}, vs fixed rule "
+             f"{ev.get('geomean_speedup_vs_heuristic', 0):.3f}")
+    with open(eval_path, "w") as fh:
+        json.dump(report, fh, indent=1)
     return 0
 
 
@@ -259,7 +308,7 @@ def eval_f64(args):
     if sel is None:
         raise SystemExit("no packaged SpMV model to evaluate")
     ev, _ = evaluate(sel, db, SPMV_DEFAULT_INDEX, device,
-                     lambda nc, nr, z: spmv_heuristic_index(nc, nr, z, item=8))
+                     lambda nc, nr, z, local: spmv_heuristic_index(nc, nr, z, item=8, local=local))
     eval_path = os.path.join(args.out, "eval.json")
     report = {}
     if os.path.exists(eval_path):

@@ -256,6 +256,38 @@ def test_sliced_combine_row_masks_equal_row_ids(dtype, use_opt, n_rows):
         assert torch.equal(y1, y2), slice_cols
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
+@pytest.mark.parametrize("pattern", ["banded", "random"])
+def test_fixed_rule_large_x(orc, pattern, dtype):
+    """The fixed rule on a matrix whose x exceeds 8 MB (2.5M columns): a
+    sample of rows finds banded columns local (plain entry tiles of 2048)
+    and random columns not (x-window tiles of 4096); both paths match the
+    oracle."""
+    rng = np.random.default_rng(21)
+    n = 2_500_000
+    if pattern == "banded":
+        off = np.arange(-2, 3)
+        ip = np.arange(0, 5 * n + 1, 5, dtype=np.int64)
+        cols = np.clip(np.repeat(np.arange(n), 5) + np.tile(off, n), 0, n - 1)
+    else:
+        lens = rng.integers(0, 8, n)
+        ip = np.zeros(n + 1, np.int64)
+        np.cumsum(lens, out=ip[1:])
+        cols = rng.integers(0, n, int(ip[-1]))
+    cols = cols.astype(np.int32)
+    npd = np.float32 if dtype == torch.float32 else np.float64
+    vals = rng.uniform(-1, 1, cols.size).astype(npd)
+    x = rng.uniform(-1, 1, n).astype(npd)
+    y64, cond = orc.spmv(ip, cols, vals, x), orc.spmv_abs(ip, cols, vals, x)
+    p = prepare_spmv_csr(to_dev(vals), to_dev(cols), to_dev(ip), (n, n), schedule="default")
+    want = (Schedule(tile_nnz=2048, tile_mode=1) if pattern == "banded"
+            else Schedule(tile_nnz=4096, tile_mode=1, x_window=-1))
+    assert p.schedule == want
+    xd = to_dev(x)
+    _check(flagsparse_spmv_csr(x=xd, prepared=p, use_opt=True), y64, cond, dtype, "opt")
+    _check(flagsparse_spmv_csr(x=xd, prepared=p), y64, cond, dtype, "default")
+
+
 def test_spmv_long_rows_compensated_fp32(orc):
     """Rows far above 4096 entries with heavy cancellation stay within 1e-5 of
     sum|a x| on the native fp32 path (compensated chunk accumulation)."""

@@ -225,8 +225,63 @@ def test_default_schedules():
     assert d.x_slice_cols == 512 * 1024 and d.tile_mode == 1 and d.tile_nnz == 2048
     assert rowsplit.default_spmv_schedule(10**6, 10, torch.float32, 10**6).x_slice_cols == 0
     assert rowsplit.default_spmv_schedule(10, 10, torch.float64, 1_100_000).x_slice_cols == 256 * 1024
+    # with the matrix's arrays the rule samples column locality (3M columns: 12 MB of fp32 x)
+    n = 3_000_000
+    r = torch.arange(n, dtype=torch.int64)
+    band_ptr = torch.arange(0, 3 * n + 1, 3, dtype=torch.int64)
+    band_cols = torch.clamp(r.repeat_interleave(3) + torch.tensor([-1, 0, 1]).repeat(n), 0, n - 1)
+    rand_ptr = torch.arange(n + 1, dtype=torch.int64)
+    rand_cols = torch.randint(0, n, (n,), generator=torch.Generator().manual_seed(3))
+    for vt in (torch.float32, torch.float64):
+        d = rowsplit.default_spmv_schedule(n, 3 * n, vt, n, band_ptr, band_cols.to(torch.int32))
+        assert d == rowsplit.Schedule(tile_nnz=2048, tile_mode=1)
+        d = rowsplit.default_spmv_schedule(n, n, vt, n, rand_ptr, rand_cols)
+        assert d == rowsplit.Schedule(tile_nnz=4096, tile_mode=1, x_window=-1)
+    # values without x-window kernels keep the column-sliced layout for random columns
+    d = rowsplit.default_spmv_schedule(n, n, torch.complex64, n, rand_ptr, rand_cols)
+    assert d.x_slice_cols == 256 * 1024 and d.x_window == 0
     r = rowsplit.row_tile_spmv_schedule(10, 10, torch.complex64)
     assert r.tile_nnz == 1024 and r.short_max <= r.tile_nnz and r.chunk_nnz % 64 == 0
+
+
+def test_far_column_fraction_brute_force():
+    """Against a per-row loop over the same sampled rows: empty rows are
+    skipped, the diagonal of a rectangular matrix is row * n_cols // n_rows."""
+    rng = np.random.default_rng(11)
+    n_rows, n_cols = 50_000, 170_000
+    lens = rng.integers(0, 4, n_rows)
+    ptr = np.zeros(n_rows + 1, np.int64)
+    np.cumsum(lens, out=ptr[1:])
+    cols = rng.integers(0, n_cols, int(ptr[-1]))
+    far_cols = 20_000
+    got = rowsplit.far_column_fraction(torch.from_numpy(ptr), torch.from_numpy(cols), n_rows,
+                                       n_cols, far_cols)
+    step = max(1, n_rows // rowsplit.LOCALITY_SAMPLE_ROWS)
+    far = tot = 0
+    for i in range(0, n_rows, step):
+        if ptr[i + 1] > ptr[i]:
+            tot += 1
+            far += abs(int(cols[ptr[i]]) - i * n_cols // n_rows) > far_cols
+    assert got == far / tot
+    assert rowsplit.far_column_fraction(torch.zeros(3, dtype=torch.int64),
+                                        torch.zeros(0, dtype=torch.int32), 2, 2, 1) is None
+
+
+@pytest.mark.parametrize("family", synthetic.SELECTOR_FAMILIES)
+def test_columns_local_by_family(family):
+    """The locality sample separates the selector's families: banded and
+    block-diagonal rows start at their diagonal, the random families'
+    (uniform, power-law, R-MAT) first columns are spread over the row range.
+    Checked at n = 200,000 with a 4,096-column band; at the large-matrix
+    supplement's sizes (2M-5M rows, 32,768 columns) the sampled fractions
+    are 0.000 for the local families and 0.97-0.99 for the others."""
+    n = 200_000
+    ip, cols, _, _ = synthetic.family_csr_torch(family, n, 7100, "cpu")
+    far = rowsplit.far_column_fraction(ip, cols, n, n, 4096)
+    if family in synthetic.LOCAL_FAMILIES:
+        assert far == 0.0
+    else:
+        assert far > 0.9
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2])
