@@ -231,6 +231,27 @@ def _event_time(fn, steps, stream):
     return a.elapsed_time(b) * 1e3 / steps
 
 
+def _pass_times(passes, steps, stream):
+    """Average duration (us) of each pass of a sequence run `steps` times in
+    a row, each pass bracketed by HIP events on `stream`: every pass is
+    timed where it runs in a real step (after the pass before it)."""
+    import torch
+    for _ in range(3):
+        for f in passes:
+            f()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(passes) + 1)]
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    for k in range(steps):
+        evs[k][0].record(stream)
+        for i, f in enumerate(passes):
+            f()
+            evs[k][i + 1].record(stream)
+    torch.cuda.synchronize()
+    return [sum(e[i].elapsed_time(e[i + 1]) for e in evs) * 1e3 / steps
+            for i in range(len(passes))]
+
+
 def vendor_spmv(d_vals, d_cols, d_ip, d_x, n, nbytes, steps):
     """rocSPARSE CSR SpMV (adaptive algorithm, analysis done once) on the
     same matrix and x: the GPU vendor baseline."""
@@ -837,10 +858,9 @@ def main(argv=None):
         # (+ the run table and tile_run for a window-major xg)
         p2_bytes = ((n + 1) * 4 + nnz * (2 * s_v + 2) + n * s_v
                     + (run_table + (xw.n_tiles + 1) * 4 if xw.order == "window" else 0))
-        g_us = _event_time(lambda: _lib.xwin_gather(xw.layout, d_x, xw.xg, dev), args.steps,
-                           stream)
-        p2_us = _event_time(lambda: _lib.spmv(xw.csr, xw.xg, yk, 1.0, 0.0, pst, dev),
-                            args.steps, stream)
+        g_us, p2_us = _pass_times([lambda: _lib.xwin_gather(xw.layout, d_x, xw.xg, dev),
+                                   lambda: _lib.spmv(xw.csr, xw.xg, yk, 1.0, 0.0, pst, dev)],
+                                  args.steps, stream)
         g_name = "%s (x windows of %d columns in LDS, %s-major xg)" % (
             "xwin_gather_wm_kernel" if xw.order == "window" else "xwin_gather_kernel",
             xw.window, xw.order)
@@ -850,15 +870,10 @@ def main(argv=None):
         for k in kernels:
             k["GB/s"] = round(k["algorithmic_bytes"] / k["us"] / 1e3, 1)
             k["frac"] = round(k["GB/s"] / HBM_PEAK_GBS, 4)
-        # the dominant kernel (the longer pass) is timed again below as `kernel`
-        if p2_us >= g_us:
-            def kernel():
-                _lib.spmv(xw.csr, xw.xg, yk, 1.0, 0.0, pst, dev)
-            kbytes, kname = p2_bytes, p2_name
-        else:
-            def kernel():
-                _lib.xwin_gather(xw.layout, d_x, xw.xg, dev)
-            kbytes, kname = g_bytes, g_name
+        # the dominant kernel is the longer pass, timed inside the sequence
+        kernel = None
+        kernel_us, kbytes, kname = ((p2_us, p2_bytes, p2_name) if p2_us >= g_us
+                                    else (g_us, g_bytes, g_name))
         nc = None
     elif sliced is not None:
         kplan = sliced.plan.struct(1)
@@ -873,17 +888,8 @@ def main(argv=None):
         nc = None
         kbytes = nbytes
         kname = "spmv_entry_kernel + carry fix-up"
-    for _ in range(3):
-        kernel()
-    k0 = torch.cuda.Event(enable_timing=True)
-    k1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    k0.record(stream)
-    for _ in range(args.steps):
-        kernel()
-    k1.record(stream)
-    torch.cuda.synchronize()
-    kernel_us = k0.elapsed_time(k1) * 1e3 / args.steps
+    if kernel is not None:
+        kernel_us = _event_time(kernel, args.steps, stream)
     kgbs = kbytes / (kernel_us * 1e-6) / 1e9
     _log(f"dominant kernel: {kernel_us:.2f} us/launch, {kgbs:.1f} GB/s over {kbytes} bytes")
     if kernels is not None:
