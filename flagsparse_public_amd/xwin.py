@@ -74,7 +74,7 @@ def default_window(value_dtype, device) -> int:
 
 
 def _as_int16(t: torch.Tensor) -> torch.Tensor:
-    """int64 values in [0, 65536) stored as their 16-bit pattern."""
+    """Integer values in [0, 65536) stored as their 16-bit pattern."""
     return torch.where(t >= 32768, t - 65536, t).to(torch.int16)
 
 
@@ -86,15 +86,19 @@ def run_table(is_start: torch.Tensor, off_at_start: torch.Tensor):
     when position 64 b + i starts a run."""
     dev, n = is_start.device, is_start.numel()
     nb = -(-n // 64)
-    run_id = torch.cumsum(is_start.to(torch.int64), 0) - 1
-    batch_run = run_id[torch.arange(nb, device=dev, dtype=torch.int64) * 64] \
-        .to(torch.int32).contiguous()
-    del run_id
-    pad = torch.zeros(nb * 64, dtype=torch.int64, device=dev)
-    pad[:n] = is_start.to(torch.int64)
-    bits = torch.bitwise_left_shift(pad.view(nb, 64),
-                                     torch.arange(64, device=dev, dtype=torch.int64))
-    batch_mask = bits.sum(dim=1).contiguous()  # distinct bits: the sum is the OR
+    pad = torch.zeros(nb * 64, dtype=torch.uint8, device=dev)
+    pad[:n] = is_start
+    rows = pad.view(nb, 64)
+    # run of position 64 b: the run starts before batch b, plus one if 64 b
+    # starts a run, minus one (runs count from 0)
+    starts = rows.sum(dim=1)
+    batch_run = (torch.cumsum(starts, 0) - starts + rows[:, 0] - 1).to(torch.int32).contiguous()
+    # bit 8 k + j of batch b from byte j of group k: pack 8 flags into a byte,
+    # then 8 bytes into the word (distinct bits: each sum is the OR)
+    b8 = torch.bitwise_left_shift(rows.view(nb, 8, 8).to(torch.int32),
+                                  torch.arange(8, device=dev, dtype=torch.int32)).sum(dim=2)
+    batch_mask = torch.bitwise_left_shift(
+        b8, torch.arange(0, 64, 8, device=dev, dtype=torch.int64)).sum(dim=1).contiguous()
     return batch_run, batch_mask, off_at_start.to(torch.int32).contiguous()
 
 
@@ -123,6 +127,17 @@ def part_ranges(win_start, nnz: int, n_parts: int):
             cuts.append(c)
     pw = [bisect.bisect_right(ws, c) - 1 for c in cuts]
     return pw, cuts + [nnz]
+
+
+def _narrow_key(key: torch.Tensor, bound: int) -> torch.Tensor:
+    """Nonnegative sort keys below ``bound`` in the narrowest signed integer
+    type that holds them: a device radix sort makes one pass per key byte,
+    and a stable sort orders equal keys the same whatever their type."""
+    if bound <= 2**15:
+        return key.to(torch.int16)
+    if bound <= 2**31:
+        return key.to(torch.int32)
+    return key
 
 
 class XWinCsr:
@@ -184,26 +199,30 @@ class XWinCsr:
             self.n_parts = 0
             self.layout = self._struct()
             return
-        cols = indices.to(torch.int64)
+        # index arithmetic in int32 wherever the values fit (entry positions
+        # do: nnz < 2^31), so each pass over the entries moves half the bytes
+        it = torch.int32 if self.n_cols <= 2**31 - 1 else torch.int64
+        cols = indices.to(it)
         win = cols // W
-        tile = torch.repeat_interleave(torch.arange(n_tiles, device=dev, dtype=torch.int64),
+        tile = torch.repeat_interleave(torch.arange(n_tiles, device=dev, dtype=torch.int32),
                                        te[1:] - te[:-1])
         # window-major order (window, tile, CSR order): entries are already in
         # tile order, so a stable sort by window is enough
-        order1 = torch.argsort(win, stable=True)
+        order1 = torch.argsort(_narrow_key(win, S), stable=True)
         self.wcol = _as_int16((cols - win * W)[order1]).contiguous()
         del cols
         # pass-2 layout: by (tile, window, CSR order)
-        key2 = tile * S + win
-        order2 = torch.argsort(key2, stable=True)
-        pos2 = torch.empty(nnz, dtype=torch.int64, device=dev)
-        pos2[order2] = torch.arange(nnz, device=dev, dtype=torch.int64)
-        self.x_perm = _as_int16(pos2 - te[tile]).contiguous()
+        kt = torch.int32 if n_tiles * S <= 2**31 - 1 else torch.int64
+        key2 = tile.to(kt) * S + win.to(kt)
+        order2 = torch.argsort(_narrow_key(key2, n_tiles * S), stable=True)
+        pos2 = torch.empty(nnz, dtype=torch.int32, device=dev)
+        pos2[order2] = torch.arange(nnz, device=dev, dtype=torch.int32)
+        self.x_perm = _as_int16(pos2 - te.to(torch.int32)[tile]).contiguous()
         # runs: the (window, tile) groups, contiguous in both orders
         if self.order == "tile":
             # over the window-major entries k: entry k goes to xg[k + run_off]
             del order2, key2
-            key1 = (win * n_tiles + tile)[order1]
+            key1 = (win.to(kt) * n_tiles + tile.to(kt))[order1]
             del tile
             is_start = torch.ones(nnz, dtype=torch.bool, device=dev)
             is_start[1:] = key1[1:] != key1[:-1]
@@ -222,8 +241,8 @@ class XWinCsr:
             is_start[1:] = key2[1:] != key2[:-1]
             del key2
             run_d0 = torch.nonzero(is_start, as_tuple=False).flatten()
-            pos1 = torch.empty(nnz, dtype=torch.int64, device=dev)
-            pos1[order1] = torch.arange(nnz, device=dev, dtype=torch.int64)
+            pos1 = torch.empty(nnz, dtype=torch.int32, device=dev)
+            pos1[order1] = torch.arange(nnz, device=dev, dtype=torch.int32)
             src = pos1[order2[run_d0]]
             del pos1, order2
             self.batch_run, self.batch_mask, self.run_off = run_table(is_start, src - run_d0)
@@ -236,7 +255,7 @@ class XWinCsr:
         self.n_runs = int(self.run_off.numel())
         # window ranges in window-major order -> gather parts
         win_sorted = win[order1]
-        ws = torch.searchsorted(win_sorted, torch.arange(S + 1, device=dev, dtype=torch.int64))
+        ws = torch.searchsorted(win_sorted, torch.arange(S + 1, device=dev, dtype=win.dtype))
         del win_sorted, win, order1
         self.window_entry = ws.contiguous()
         item = torch.empty(0, dtype=self.value_dtype).element_size()
