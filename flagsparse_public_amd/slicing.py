@@ -27,6 +27,7 @@ import torch
 
 from . import _lib
 from .rowsplit import RowSplitPlan, Schedule
+from .sortkey import stable_order
 
 N_GROUPS = 8          # MI355X: 8 XCDs, workgroups dealt round-robin
 COMBINE_ROWS = 256    # rows per block of the combine pass
@@ -84,7 +85,7 @@ class SlicedCsr:
         sl = indices.to(torch.int64) // w
         key = sl * n + rows
         del rows
-        order = torch.argsort(key, stable=True)
+        order = stable_order(key, P * n)
         key = key[order]
         self.cols = indices[order].contiguous()
         self.vals = data[order].contiguous() if data is not None else None

@@ -18,6 +18,7 @@ import time
 
 import torch
 
+from ..sortkey import stable_order
 from ._common import SUPPORTED_INDEX_DTYPES, _INDEX_LIMIT_INT32
 from .spmm_csr import SUPPORTED_SPMM_VALUE_DTYPES, _select_spmm_alg1_warp_and_factor
 
@@ -97,8 +98,9 @@ def _sorted_csr(data, row, col, n_rows):
     """Stable (row, col) order and the row pointer of the sorted entries."""
     if data.numel() == 0:
         return data, col, torch.zeros(n_rows + 1, dtype=torch.int64, device=data.device)
-    key = row.to(torch.int64) * (int(col.max().item()) + 1) + col.to(torch.int64)
-    order = torch.argsort(key, stable=True)
+    width = int(col.max().item()) + 1
+    key = row.to(torch.int64) * width + col.to(torch.int64)
+    order = stable_order(key, n_rows * width)
     counts = torch.bincount(row.to(torch.int64), minlength=n_rows)
     indptr = torch.zeros(n_rows + 1, dtype=torch.int64, device=data.device)
     torch.cumsum(counts, 0, out=indptr[1:])

@@ -26,6 +26,7 @@ from .. import _lib
 from ..rowsplit import RowSplitPlan, Schedule, default_spmv_schedule, supported_spmv_schedule
 from ..schedule_choice import choose_schedule, exhaustive_spmv_schedule
 from ..slicing import SLICED_DTYPES, SlicedCsr
+from ..sortkey import stable_order
 from ..xwin import XWIN_DTYPES, XWinCsr
 from ._common import (
     SUPPORTED_INDEX_DTYPES,
@@ -253,7 +254,7 @@ def _transpose_csr_for_spmv(data, indices, indptr, shape):
     counts = (indptr[1:] - indptr[:-1]).to(torch.int64)
     row_of = torch.repeat_interleave(torch.arange(n_rows, dtype=torch.int64, device=device), counts)
     cols = indices.to(torch.int64)
-    order = torch.argsort(cols, stable=True)
+    order = stable_order(cols, n_cols)
     t_indptr = torch.zeros(n_cols + 1, dtype=torch.int64, device=device)
     t_indptr[1:] = torch.cumsum(torch.bincount(cols, minlength=n_cols), dim=0)
     return (data[order].contiguous(), row_of[order].to(out_index_dtype).contiguous(),

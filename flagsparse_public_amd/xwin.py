@@ -56,6 +56,7 @@ import torch
 
 from . import _lib
 from .rowsplit import RowSplitPlan, Schedule
+from .sortkey import stable_order
 
 XWIN_DTYPES = (torch.float32, torch.float64)
 DEFAULT_WINDOW_BYTES = 128 * 1024
@@ -129,17 +130,6 @@ def part_ranges(win_start, nnz: int, n_parts: int):
     return pw, cuts + [nnz]
 
 
-def _narrow_key(key: torch.Tensor, bound: int) -> torch.Tensor:
-    """Nonnegative sort keys below ``bound`` in the narrowest signed integer
-    type that holds them: a device radix sort makes one pass per key byte,
-    and a stable sort orders equal keys the same whatever their type."""
-    if bound <= 2**15:
-        return key.to(torch.int16)
-    if bound <= 2**31:
-        return key.to(torch.int32)
-    return key
-
-
 class XWinCsr:
     """The x-window layout of one CSR matrix, its gathered-x buffer and the
     second pass's entry-tile plans (one per accumulation mode)."""
@@ -208,13 +198,13 @@ class XWinCsr:
                                        te[1:] - te[:-1])
         # window-major order (window, tile, CSR order): entries are already in
         # tile order, so a stable sort by window is enough
-        order1 = torch.argsort(_narrow_key(win, S), stable=True)
+        order1 = stable_order(win, S)
         self.wcol = _as_int16((cols - win * W)[order1]).contiguous()
         del cols
         # pass-2 layout: by (tile, window, CSR order)
         kt = torch.int32 if n_tiles * S <= 2**31 - 1 else torch.int64
         key2 = tile.to(kt) * S + win.to(kt)
-        order2 = torch.argsort(_narrow_key(key2, n_tiles * S), stable=True)
+        order2 = stable_order(key2, n_tiles * S)
         pos2 = torch.empty(nnz, dtype=torch.int32, device=dev)
         pos2[order2] = torch.arange(nnz, device=dev, dtype=torch.int32)
         self.x_perm = _as_int16(pos2 - te.to(torch.int32)[tile]).contiguous()

@@ -530,3 +530,18 @@ def test_supported_spmv_schedule():
     sliced = Schedule(tile_nnz=1024, tile_mode=1, x_slice_cols=1 << 18)
     out = supported_spmv_schedule(sliced, torch.float32, torch.int64)
     assert out.tile_nnz == 2048 and out.x_slice_cols == 1 << 18
+
+
+def test_stable_order_narrow_keys():
+    """Stable orders of keys narrowed to 16 / 32 bits equal the int64 order,
+    equal keys in input order (every bound class, duplicates included)."""
+    from flagsparse_public_amd.sortkey import narrow_key, stable_order
+    g = torch.Generator().manual_seed(9)
+    for bound in (7, 2**15, 2**15 + 1, 2**31, 2**31 + 5):
+        key = torch.randint(0, min(bound, 2**40), (20000,), generator=g, dtype=torch.int64)
+        key[::7] = bound - 1
+        want = torch.argsort(key, stable=True)
+        assert torch.equal(stable_order(key, bound), want), bound
+        nk = narrow_key(key, bound)
+        assert nk.dtype == (torch.int16 if bound <= 2**15 else
+                            torch.int32 if bound <= 2**31 else torch.int64)
