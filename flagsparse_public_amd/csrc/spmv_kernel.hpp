@@ -501,7 +501,11 @@ void spmv_launch_tiles(const SpmvParams<VS, V, I, P>& p, int64_t grid, hipStream
 // straddling row's carries in tile order.  Rows of any length take the same
 // path; results depend on T only through where a long row is cut.
 
-constexpr int kSegRows = 1024;  // row boundaries staged in LDS per pass
+// Row boundaries staged in LDS per pass.  768 (not 1024) keeps a 4096-entry
+// fp32 tile's LDS (16 KB of products + these slots) within 20 KB, so 8
+// blocks -- the 32-wave limit -- fit a CU instead of 7: on the 4M-row
+// power-law SpMV the gathered-x pass went from 104 to 100 us.
+constexpr int kSegRows = 768;
 // Rows longer than kThreadRowMax inside one tile: at most tile_nnz / 33 (124
 // for the largest tile, 4096 entries), listed in LDS for the wavefronts.
 constexpr int kLongRowsMax = 128;

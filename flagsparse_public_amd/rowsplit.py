@@ -228,7 +228,7 @@ class RowSplitPlan:
             self.tile_start = None
             self.n_tiles = 0
 
-    ENTRY_ROW_CAP = 1024  # owned rows per entry tile (the kernel's LDS row batch)
+    ENTRY_ROW_CAP = 768  # owned rows per entry tile (the kernel's LDS row batch)
 
     def _init_entry_tiles(self, indptr, n_rows, schedule):
         """Entry tiles: cut the entries every T and, inside a tile that owns

@@ -151,7 +151,7 @@ typedef struct fs_plan {
  *                    that range (the last tile also owns rows starting at
  *                    nnz); tile_entry[0] = 0, tile_entry[n_tiles] = nnz,
  *                    tile_start[n_tiles] = n_rows, both nondecreasing.  The
- *                    planner cuts every tile_nnz entries and every 1024 owned
+ *                    planner cuts every tile_nnz entries and every 768 owned
  *                    rows.  No long-row lists (n_long = 0); partials = two
  *                    carries per tile (fs_tile_carry_bytes).  tile_nnz is
  *                    1024, 2048 or 4096 for fp32 / fp64 values with int32

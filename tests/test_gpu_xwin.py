@@ -160,7 +160,7 @@ def test_xwin_random_shapes(case):
     n_cols = int(np.exp(rng.uniform(np.log(4), np.log(300_000))))
     lens = rng.poisson(rng.uniform(0.05, 60), n)
     if case % 3 == 0:
-        lens[: n // 2] = 1  # one-entry rows: tiles cut every 1024 rows
+        lens[: n // 2] = 1  # one-entry rows: tiles cut every 768 rows
     lens = np.minimum(lens, n_cols)
     ip = np.zeros(n + 1, np.int64)
     np.cumsum(lens, out=ip[1:])

@@ -97,7 +97,7 @@ def test_powerlaw_layout(orc, T, W, order):
 
 
 def test_row_capped_tiles_start_off_batch(orc):
-    # runs of one-entry rows cut tiles every 1024 rows at entry offsets that
+    # runs of one-entry rows cut tiles every 768 rows at entry offsets that
     # are not multiples of 64: pass 2's positions then straddle batches
     lens = np.ones(9000, np.int64)
     lens[::97] = 40
