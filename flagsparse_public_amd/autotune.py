@@ -152,7 +152,7 @@ def tune_spmv(prepared, x, variants=SPMV_VARIANTS, reps=7, warmup=2, inner=1):
     y = torch.empty(prepared.n_rows, dtype=prepared.data.dtype, device=prepared.data.device)
     times = []
     dt, it = prepared.data.dtype, prepared.kernel_indices.dtype
-    for sch in (supported_spmv_schedule(v, dt, it) for v in variants):
+    for sch in (supported_spmv_schedule(v, dt, it, prepared.data.numel()) for v in variants):
         times.append(time_launches(
             lambda s=sch: _hip_spmv_csr_impl_opt_prepared(prepared, x, out=y, schedule=s),
             reps, warmup, inner))

@@ -98,9 +98,10 @@ def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None, n_cols: int |
     the matrix's row pointers and columns are given, a sample of rows
     decides (columns_local) between plain entry tiles of 2048 (local
     columns: the x gathers hit L2) and, for random columns, the x-window
-    layout with tiles of 4096 (fp32 / fp64 values) or entry tiles of 2048
-    over the column-sliced layout (2 MB of x per slice; other values, or
-    without the matrix's arrays).  Otherwise one lane group per row, its
+    layout with tiles of 4096 (fp32 / fp64 values, up to XWIN_MAX_NNZ
+    entries) or entry tiles of 2048 over the column-sliced layout (2 MB of x
+    per slice; other values, larger matrices, or without the matrix's
+    arrays).  Otherwise one lane group per row, its
     width from the mean row length (rowgroup_spmv_schedule)."""
     if value_dtype is not None and n_cols is not None:
         item = torch.empty(0, dtype=value_dtype).element_size()
@@ -110,22 +111,28 @@ def default_spmv_schedule(n_rows: int, nnz: int, value_dtype=None, n_cols: int |
                 return sliced
             if columns_local(indptr, indices, n_rows, n_cols, item):
                 return Schedule(tile_nnz=2048, tile_mode=1)
-            if value_dtype in (torch.float32, torch.float64):
+            if value_dtype in (torch.float32, torch.float64) and nnz <= XWIN_MAX_NNZ:
                 return Schedule(tile_nnz=4096, tile_mode=1, x_window=-1)
             return sliced
     return rowgroup_spmv_schedule(n_rows, nnz)
 
 
-def supported_spmv_schedule(sch: Schedule, value_dtype, index_dtype) -> Schedule:
+XWIN_MAX_NNZ = 2**31 - 1  # entries the x-window layout addresses
+
+
+def supported_spmv_schedule(sch: Schedule, value_dtype, index_dtype,
+                            nnz: int | None = None) -> Schedule:
     """The schedule a library-chosen variant runs as for these dtypes: entry
     tiles of 1024 / 4096 entries are built for fp32 / fp64 values with int32
     columns only (others: tiles of 2048), and the x-window layout (whose second
     pass indexes through 16-bit positions, any column type) for fp32 / fp64
-    values only."""
+    values and at most XWIN_MAX_NNZ entries (``nnz``, when given) -- beyond
+    that the launch runs plain entry tiles of the same size."""
     if sch.tile_mode != 1 or sch.tile_nnz in (0, 2048):
         return sch
     fp = value_dtype in (torch.float32, torch.float64)
-    if fp and (sch.x_window != 0 or index_dtype == torch.int32):
+    xwin = sch.x_window != 0 and (nnz is None or nnz <= XWIN_MAX_NNZ)
+    if fp and (xwin or index_dtype == torch.int32):
         return sch
     return dataclasses.replace(sch, tile_nnz=2048)
 

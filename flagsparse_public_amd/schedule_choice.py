@@ -66,4 +66,4 @@ def exhaustive_spmv_schedule(prepared, reps=5):
     from .rowsplit import supported_spmv_schedule
     times = tune_spmv(prepared, x, reps=reps)
     return supported_spmv_schedule(best_schedule(times, SPMV_VARIANTS)[1], prepared.data.dtype,
-                                   prepared.kernel_indices.dtype)
+                                   prepared.kernel_indices.dtype, prepared.data.numel())

@@ -365,7 +365,7 @@ def prepare_spmv_csr(data, indices, indptr, shape, block_nnz=256, max_segments=N
     sch, how = choose_schedule("spmv", schedule, data, n_rows, n_cols, stats_out[0][0],
                                kernel_indptr, kernel_indices)
     if how != "given":  # a library choice runs as what is built for these dtypes
-        sch = supported_spmv_schedule(sch, data.dtype, kernel_indices.dtype)
+        sch = supported_spmv_schedule(sch, data.dtype, kernel_indices.dtype, data.numel())
     prepared = PreparedCsrSpmv(
         data=data, kernel_indices=kernel_indices, kernel_indptr=kernel_indptr, shape=shape,
         n_rows=n_rows, n_cols=n_cols, block_nnz=block_nnz_use, max_segments=max_segments_use,

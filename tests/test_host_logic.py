@@ -240,6 +240,9 @@ def test_default_schedules():
     # values without x-window kernels keep the column-sliced layout for random columns
     d = rowsplit.default_spmv_schedule(n, n, torch.complex64, n, rand_ptr, rand_cols)
     assert d.x_slice_cols == 256 * 1024 and d.x_window == 0
+    # so do matrices beyond the x-window layout's 2^31 - 1 entries
+    d = rowsplit.default_spmv_schedule(n, 2**31, torch.float32, n, rand_ptr, rand_cols)
+    assert d.x_slice_cols == 512 * 1024 and d.x_window == 0
     r = rowsplit.row_tile_spmv_schedule(10, 10, torch.complex64)
     assert r.tile_nnz == 1024 and r.short_max <= r.tile_nnz and r.chunk_nnz % 64 == 0
 
@@ -525,6 +528,10 @@ def test_supported_spmv_schedule():
     assert supported_spmv_schedule(t4, torch.float64, torch.int64).tile_nnz == 2048
     assert supported_spmv_schedule(t4, torch.float16, torch.int32).tile_nnz == 2048
     assert supported_spmv_schedule(xw, torch.float32, torch.int64) == xw
+    assert supported_spmv_schedule(xw, torch.float32, torch.int64, 2**31 - 1) == xw
+    # beyond the x-window layout's reach the launch runs plain entry tiles
+    assert supported_spmv_schedule(xw, torch.float32, torch.int64, 2**31).tile_nnz == 2048
+    assert supported_spmv_schedule(xw, torch.float32, torch.int32, 2**31) == xw
     assert supported_spmv_schedule(xw, torch.complex64, torch.int32).tile_nnz == 2048
     assert supported_spmv_schedule(rows, torch.float16, torch.int64) == rows
     sliced = Schedule(tile_nnz=1024, tile_mode=1, x_slice_cols=1 << 18)
