@@ -256,13 +256,14 @@ def test_sliced_combine_row_masks_equal_row_ids(dtype, use_opt, n_rows):
         assert torch.equal(y1, y2), slice_cols
 
 
+@pytest.mark.parametrize("index_dtype", [np.int32, np.int64], ids=["i32", "i64"])
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64], ids=["f32", "f64"])
 @pytest.mark.parametrize("pattern", ["banded", "random"])
-def test_fixed_rule_large_x(orc, pattern, dtype):
+def test_fixed_rule_large_x(orc, pattern, dtype, index_dtype):
     """The fixed rule on a matrix whose x exceeds 8 MB (2.5M columns): a
     sample of rows finds banded columns local (plain entry tiles of 2048)
-    and random columns not (x-window tiles of 4096); both paths match the
-    oracle."""
+    and random columns not (x-window tiles of 4096, any column type); both
+    paths match the oracle."""
     rng = np.random.default_rng(21)
     n = 2_500_000
     if pattern == "banded":
@@ -274,7 +275,7 @@ def test_fixed_rule_large_x(orc, pattern, dtype):
         ip = np.zeros(n + 1, np.int64)
         np.cumsum(lens, out=ip[1:])
         cols = rng.integers(0, n, int(ip[-1]))
-    cols = cols.astype(np.int32)
+    cols = cols.astype(index_dtype)
     npd = np.float32 if dtype == torch.float32 else np.float64
     vals = rng.uniform(-1, 1, cols.size).astype(npd)
     x = rng.uniform(-1, 1, n).astype(npd)
