@@ -734,6 +734,23 @@ def _selection_record(times, learned, default, rule, how):
     return rec
 
 
+def device_stream_rates(mib=1024, reps=20):
+    """What a plain streaming kernel reaches on this GPU: a device copy of
+    `mib` MiB of fp32 (bytes read + written), averaged over `reps` launches
+    timed with HIP events."""
+    import torch
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, device=dev)
+    dst = torch.empty_like(src)
+    copy_us = _event_time(lambda: dst.copy_(src), reps, stream)
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"copy_GBps": round(2 * n * 4 / copy_us / 1e3, 1), "copy_bytes": 2 * n * 4,
+            "reps": reps}
+
+
 def calibrate_copy(mib, reps):
     """Plain device-to-device copies of a known size: the reference point that
     turns rocprofv3 FETCH_SIZE / WRITE_SIZE readings into bytes."""
@@ -940,6 +957,11 @@ def main(argv=None):
     }
     if kernels is not None:
         result["roofline"]["kernels"] = kernels
+    # the rates plain streaming kernels reach on this GPU, beside the
+    # published peak: how close the dominant kernel is to what is reachable
+    streams = device_stream_rates()
+    streams["kernel_frac_of_copy"] = round(kgbs / streams["copy_GBps"], 4)
+    result["roofline"]["device_stream"] = streams
     # the second bound of a gather-heavy SpMV: L2 requests per launch (TCC_REQ,
     # rocprofv3, committed summary) over the launch time measured here
     reqs = (_profile_value("spmv_powerlaw_4m_xwin_pmc.txt" if xw is not None
