@@ -23,3 +23,10 @@ def narrow_key(key: torch.Tensor, bound: int) -> torch.Tensor:
 def stable_order(key: torch.Tensor, bound: int) -> torch.Tensor:
     """Indices that sort ``key`` (values in [0, bound)) stably."""
     return torch.argsort(narrow_key(key, bound), stable=True)
+
+
+def stable_sort(key: torch.Tensor, bound: int):
+    """(sorted keys in the narrowed type, indices) of a stable sort of
+    ``key`` (values in [0, bound)): the sorted keys come with the sort, not
+    from a gather afterwards."""
+    return torch.sort(narrow_key(key, bound), stable=True)

@@ -56,7 +56,7 @@ import torch
 
 from . import _lib
 from .rowsplit import RowSplitPlan, Schedule
-from .sortkey import stable_order
+from .sortkey import stable_order, stable_sort
 
 XWIN_DTYPES = (torch.float32, torch.float64)
 DEFAULT_WINDOW_BYTES = 128 * 1024
@@ -75,8 +75,9 @@ def default_window(value_dtype, device) -> int:
 
 
 def _as_int16(t: torch.Tensor) -> torch.Tensor:
-    """Integer values in [0, 65536) stored as their 16-bit pattern."""
-    return torch.where(t >= 32768, t - 65536, t).to(torch.int16)
+    """Integer values in [0, 65536) stored as their 16-bit pattern (the
+    integer narrowing conversion keeps the low 16 bits)."""
+    return t.to(torch.int16)
 
 
 def run_table(is_start: torch.Tensor, off_at_start: torch.Tensor):
@@ -203,8 +204,7 @@ class XWinCsr:
         del cols
         # pass-2 layout: by (tile, window, CSR order)
         kt = torch.int32 if n_tiles * S <= 2**31 - 1 else torch.int64
-        key2 = tile.to(kt) * S + win.to(kt)
-        order2 = stable_order(key2, n_tiles * S)
+        key2, order2 = stable_sort(tile.to(kt) * S + win.to(kt), n_tiles * S)
         pos2 = torch.empty(nnz, dtype=torch.int32, device=dev)
         pos2[order2] = torch.arange(nnz, device=dev, dtype=torch.int32)
         self.x_perm = _as_int16(pos2 - te.to(torch.int32)[tile]).contiguous()
@@ -226,15 +226,19 @@ class XWinCsr:
         else:
             # over the tile-order positions d: position d reads xg[d + run_off]
             del tile, pos2
-            key2 = key2[order2]
             is_start = torch.ones(nnz, dtype=torch.bool, device=dev)
             is_start[1:] = key2[1:] != key2[:-1]
-            del key2
             run_d0 = torch.nonzero(is_start, as_tuple=False).flatten()
-            pos1 = torch.empty(nnz, dtype=torch.int32, device=dev)
-            pos1[order1] = torch.arange(nnz, device=dev, dtype=torch.int32)
-            src = pos1[order2[run_d0]]
-            del pos1, order2
+            # each run's first position in xg: window-major order lays the
+            # runs out by (window, tile), so a run starts after the runs that
+            # sort before it -- an exclusive sum of run lengths in that order
+            rkey = key2[run_d0].to(torch.int64)
+            del key2, order2
+            lens = torch.diff(run_d0, append=torch.full((1,), nnz, dtype=torch.int64, device=dev))
+            o = torch.argsort((rkey % S) * n_tiles + rkey // S)
+            src = torch.empty_like(run_d0)
+            src[o] = torch.cumsum(lens[o], 0) - lens[o]
+            del rkey, lens, o
             self.batch_run, self.batch_mask, self.run_off = run_table(is_start, src - run_d0)
             del src, is_start
             # tile t's runs: those starting in [te[t], te[t+1]) (runs never
@@ -243,10 +247,11 @@ class XWinCsr:
             del run_d0
             self.xg_runs = (self.tile_run, self.batch_run, self.batch_mask, self.run_off)
         self.n_runs = int(self.run_off.numel())
-        # window ranges in window-major order -> gather parts
-        win_sorted = win[order1]
-        ws = torch.searchsorted(win_sorted, torch.arange(S + 1, device=dev, dtype=win.dtype))
-        del win_sorted, win, order1
+        # window ranges in window-major order (entries per window, summed up)
+        # -> gather parts
+        ws = torch.zeros(S + 1, dtype=torch.int64, device=dev)
+        torch.cumsum(torch.bincount(win, minlength=S), 0, out=ws[1:])
+        del win, order1
         self.window_entry = ws.contiguous()
         item = torch.empty(0, dtype=self.value_dtype).element_size()
         pw, pe = part_ranges(ws.cpu().tolist(), nnz, gather_parts(dev, W * item))
