@@ -146,7 +146,7 @@ def test_xwin_powerlaw_4m_full_size(orc):
         _check(y, y64, cond, torch.float32, use_opt)
 
 
-@pytest.mark.parametrize("case", range(16))
+@pytest.mark.parametrize("case", range(64))
 def test_xwin_random_shapes(case):
     """Seeded random shapes: sizes from a few rows to 60k, densities from
     mostly-empty to ~60 entries per row with runs of one-entry rows (row-cap
