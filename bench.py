@@ -780,6 +780,9 @@ def main(argv=None):
     ap.add_argument("--no-vendor", action="store_true", help="skip the rocSPARSE line")
     ap.add_argument("--quick", action="store_true",
                     help="only the headline SpMV line (no nested workloads)")
+    ap.add_argument("--schedule", choices=["learned", "sliced"], default="learned",
+                    help="the headline SpMV's schedule: the library's choice (default) or entry "
+                         "tiles of 2048 over the column-sliced layout (2 MB of x per slice)")
     ap.add_argument("--workload", default="spmv_powerlaw_4m",
                     choices=["spmv_powerlaw_4m", "spmm_banded_1m", "batched_512",
                              "sddmm_poisson_1m"],
@@ -827,7 +830,10 @@ def main(argv=None):
     d_cols = torch.from_numpy(cols).to(dev)
     d_vals = torch.from_numpy(vals).to(dev)
     d_x = torch.from_numpy(x).to(dev)
-    prepared = prepare_spmv_csr(d_vals, d_cols, d_ip, (n, n))
+    from flagsparse_public_amd.rowsplit import L2_SLICE_BYTES, Schedule
+    sched = (None if args.schedule == "learned" else
+             Schedule(tile_nnz=2048, tile_mode=1, x_slice_cols=L2_SLICE_BYTES // s_v))
+    prepared = prepare_spmv_csr(d_vals, d_cols, d_ip, (n, n), schedule=sched)
     use_opt = args.path == "opt"
     y = torch.empty(n, dtype=d_vals.dtype, device=dev)
 
