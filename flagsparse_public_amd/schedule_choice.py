@@ -6,9 +6,9 @@
   * "learned" (also what None means) runs the packaged selector for ``op`` on
     the device -- features from the analysis pass's statistics, MLP forward,
     argmin -- and falls back to the fixed rule when no model is installed, the
-    values are not fp32 (SpMV: fp32 or fp64; the SpMV model, trained on fp32
-    timings, was scored on held-out fp64 timings as well), or the matrix is
-    larger than any the model was trained on;
+    values are not fp32 (SpMV: fp32 or fp64; fp64 SpMV uses the model trained
+    on the same matrices timed with fp64 values), or the matrix is larger than
+    any the model was trained on;
   * "exhaustive" is resolved after the prepared object exists
     (``exhaustive_spmv_schedule`` times every variant).
 Returns (schedule or None, how).
@@ -46,7 +46,13 @@ def choose_schedule(op, request, data, n_rows, n_cols, stats_dev, indptr=None, i
     if data.dtype != torch.float32 and not (op == "spmv" and data.dtype == torch.float64):
         return fallback(), "default"
     from .selector import load_default
-    sel = load_default(op, data.device)
+    sel = None
+    if op == "spmv" and data.dtype == torch.float64:
+        # the model trained on fp64 timings where one is packaged; else the
+        # fp32-trained one (its features describe only the sparsity structure)
+        sel = load_default(op, data.device, "f64")
+    if sel is None:
+        sel = load_default(op, data.device)
     if sel is None:
         return fallback(), "default"
     if not sel.covers(n_rows, n_cols, nnz):

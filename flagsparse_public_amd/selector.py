@@ -205,17 +205,19 @@ class Selector:
 _CACHE = {}
 
 
-def model_path(op: str) -> str:
-    return os.path.join(default_db_dir(), f"{op}_f32_mlp.json")
+def model_path(op: str, value_tag: str = "f32") -> str:
+    """Packaged model of op trained on timings with `value_tag` values
+    ("f32", or "f64" for the SpMV model trained on fp64 timings)."""
+    return os.path.join(default_db_dir(), f"{op}_{value_tag}_mlp.json")
 
 
-def load_default(op: str, device):
-    """The packaged model for op ("spmv" / "spmm64"), or None if absent or
-    trained for another variant list (a warning says so; callers then use
-    the fixed rule)."""
-    key = (op, str(device))
+def load_default(op: str, device, value_tag: str = "f32"):
+    """The packaged model for op ("spmv" / "spmm64") and value tag, or None
+    if absent or trained for another variant list (a warning says so;
+    callers then use the fixed rule)."""
+    key = (op, str(device), value_tag)
     if key not in _CACHE:
-        path = model_path(op)
+        path = model_path(op, value_tag)
         model = None
         if os.path.exists(path):
             try:

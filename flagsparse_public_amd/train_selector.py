@@ -20,12 +20,20 @@ DIR/eval.json; prints one progress line per 20 matrices.
 scores the packaged models on the held-out timings in DIR again (no
 collection, no training) and rewrites their reports in DIR/eval.json.
 
+    python -m flagsparse_public_amd.train_selector --out DIR --value-dtype float64 --ops spmv
+
+does the same for SpMV with fp64 values and x: the same matrices, timed with
+fp64 values (DIR/spmv_f64_{train,test}_db.npz), train the model the library
+uses for fp64 SpMV (DIR/spmv_f64_mlp.json, eval.json "spmv_f64").
+
     python -m flagsparse_public_amd.train_selector --out DIR --value-dtype float64 --eval-only
 
 times every SpMV variant on the held-out matrices with fp64 values
-(DIR/spmv_f64_test_db.npz) and scores the packaged fp32-trained SpMV model
-on them (eval.json "spmv_f64_held_out"): the features depend only on the
-sparsity structure, so the question is whether the best variant does too.
+(DIR/spmv_f64_test_db.npz, unless --resume finds it) and scores on them the
+packaged fp32-trained SpMV model (eval.json "spmv_f64_held_out") and the
+fp64-trained one when packaged (eval.json "spmv_f64"): the features depend
+only on the sparsity structure, so the first question is whether the best
+variant does too.
 """
 
 from __future__ import annotations
@@ -168,20 +176,23 @@ def main(argv=None):
                     help="score the packaged models without training: on the held-out timings "
                          "in DIR (float32), or on held-out matrices timed in fp64 (float64)")
     args = ap.parse_args(argv)
-    if args.value_dtype == "float64" and not args.eval_only:
-        raise SystemExit("fp64 timings are collected to evaluate the packaged model: "
-                         "use --value-dtype float64 --eval-only")
     if args.eval_only:
         return eval_f64(args) if args.value_dtype == "float64" else eval_f32(args)
     ops = [o for o in args.ops.split(",") if o]
     for o in ops:
         if o not in ("spmv", "spmm64"):
             raise SystemExit(f"unknown op {o}")
+    f64 = args.value_dtype == "float64"
+    if f64 and ops != ["spmv"]:
+        raise SystemExit("fp64 timings are collected for SpMV only: use --ops spmv")
+    tag = "f64" if f64 else "f32"
+    value_dtype = torch.float64 if f64 else torch.float32
     os.makedirs(args.out, exist_ok=True)
     device = torch.device("cuda", 0)
     name = torch.cuda.get_device_name(device)
     t0 = time.time()
-    paths = {s: {op: os.path.join(args.out, f"{op}_{s}_db.npz") for op in ops}
+    db_tag = "_f64" if f64 else ""
+    paths = {s: {op: os.path.join(args.out, f"{op}{db_tag}_{s}_db.npz") for op in ops}
              for s in ("train", "test")}
     n_variants = {"spmv": len(SPMV_VARIANTS), "spmm64": len(SPMM_VARIANTS)}
     dbs = {}
@@ -202,8 +213,8 @@ def main(argv=None):
             for op, db in have.items():
                 db.save(paths[split][op])
         if seeds and complete:
-            complete = collect(seeds, device, have, args.max_n, args.reps, t0, split, save=save,
-                               deadline=deadline)
+            complete = collect(seeds, device, have, args.max_n, args.reps, t0, f"{split} {tag}",
+                               save=save, deadline=deadline, value_dtype=value_dtype)
             save()
         dbs[split] = have
     if args.collect_only or not complete:
@@ -214,8 +225,9 @@ def main(argv=None):
     if os.path.exists(eval_path):  # keep the other operations' reports
         with open(eval_path) as fh:
             report = json.load(fh)
-    report.update({"device": name, "n_train": len(dbs["train"][ops[0]]),
-                   "n_test": len(dbs["test"][ops[0]])})
+    if not f64:
+        report.update({"device": name, "n_train": len(dbs["train"][ops[0]]),
+                       "n_test": len(dbs["test"][ops[0]])})
     defaults = {"spmv": SPMV_DEFAULT_INDEX, "spmm64": SPMM_DEFAULT_INDEX}
     for op in ops:
         default_index = defaults[op]
@@ -225,13 +237,15 @@ def main(argv=None):
         _, _, _, _, sizes = dbs["train"][op].arrays()
         sel.meta.update({"device": name, "default_index": default_index,
                          "n_max": int(sizes[:, 0].max()), "nnz_max": int(sizes[:, 1].max())})
-        heur = _heuristic_f32 if op == "spmv" else None
+        heur = (_heuristic_f64 if f64 else _heuristic_f32) if op == "spmv" else None
         ev, _ = evaluate(sel, dbs["test"][op], default_index, device, heur)
         tr, _ = evaluate(sel, dbs["train"][op], default_index, device, heur)
-        report[op] = {"held_out": ev, "train": {k: tr[k] for k in
-                                                ("geomean_regret", "geomean_speedup_vs_default")}}
+        report[op + db_tag] = {"held_out": ev, "train": {k: tr[k] for k in
+                                                         ("geomean_regret", "geomean_speedup_vs_default")},
+                               "n_train": len(dbs["train"][op]), "n_test": len(dbs["test"][op])}
         sel.meta["held_out"] = ev
-        sel.save(os.path.join(args.out, f"{op}_f32_mlp.json"))
+        sel.meta["value_dtype"] = args.value_dtype
+        sel.save(os.path.join(args.out, f"{op}_{tag}_mlp.json"))
         _log(f"{op}: held-out regret {ev['geomean_regret']:.3f}, "
              f"speed-up vs default {ev['geomean_speedup_vs_default']:.3f}")
     with open(eval_path, "w") as fh:
@@ -242,6 +256,10 @@ def main(argv=None):
 
 def _heuristic_f32(n_cols, n_rows, nnz, local):
     return spmv_heuristic_index(n_cols, n_rows, nnz, local=local)
+
+
+def _heuristic_f64(n_cols, n_rows, nnz, local):
+    return spmv_heuristic_index(n_cols, n_rows, nnz, item=8, local=local)
 
 
 def eval_f32(args):
@@ -283,8 +301,9 @@ def eval_f32(args):
 
 
 def eval_f64(args):
-    """Held-out matrices with fp64 values: time every SpMV variant, score the
-    packaged fp32-trained model (and the fp64 fixed rule) on them."""
+    """Held-out matrices with fp64 values: time every SpMV variant (or reuse
+    the timings with --resume), score the packaged fp32-trained model, the
+    fp64-trained one when packaged, and the fp64 fixed rule on them."""
     from .selector import load_default
     os.makedirs(args.out, exist_ok=True)
     device = torch.device("cuda", 0)
@@ -307,14 +326,20 @@ def eval_f64(args):
     sel = load_default("spmv", device)
     if sel is None:
         raise SystemExit("no packaged SpMV model to evaluate")
-    ev, _ = evaluate(sel, db, SPMV_DEFAULT_INDEX, device,
-                     lambda nc, nr, z, local: spmv_heuristic_index(nc, nr, z, item=8, local=local))
+    ev, _ = evaluate(sel, db, SPMV_DEFAULT_INDEX, device, _heuristic_f64)
     eval_path = os.path.join(args.out, "eval.json")
     report = {}
     if os.path.exists(eval_path):
         with open(eval_path) as fh:
             report = json.load(fh)
     report["spmv_f64_held_out"] = ev
+    sel64 = load_default("spmv", device, "f64")
+    if sel64 is not None:
+        ev64, _ = evaluate(sel64, db, SPMV_DEFAULT_INDEX, device, _heuristic_f64)
+        report.setdefault("spmv_f64", {})["held_out"] = ev64
+        _log(f"fp64-trained model: regret {ev64['geomean_regret']:.3f}, within 5% "
+             f"{ev64['frac_within_5pct_of_exhaustive']:.3f}, vs fixed rule "
+             f"{ev64.get('geomean_speedup_vs_heuristic', 0):.3f}")
     with open(eval_path, "w") as fh:
         json.dump(report, fh, indent=1)
     _log(f"fp64: regret {ev['geomean_regret']:.3f}, within 5% {ev['frac_within_5pct_of_exhaustive']:.3f}, "

@@ -2,8 +2,9 @@
 
 The timing databases (flagsparse_public_amd/selector_models/*_db.npz) hold the
 autotuner's per-variant device times for held-out matrices (seeds 7000..) and
-the training matrices (5000..); the models (*_f32_mlp.json) were trained on the
-training rows by the library's HIP MLP kernels.  Here the oracle's MLP forward
+the training matrices (5000..); the models (*_f32_mlp.json, and
+spmv_f64_mlp.json on the same matrices timed with fp64 values) were trained on
+the training rows by the library's HIP MLP kernels.  Here the oracle's MLP forward
 (oracle/csr_oracle.c) recomputes every held-out choice from the stored
 features, and the test checks
   * the selection matches the one recorded on the device at evaluation time
@@ -87,19 +88,38 @@ def test_selector_gain_on_held_out(orc, op, default_index):
         assert gain >= 0.97
 
 
-def test_spmv_model_on_fp64_held_out(orc):
-    """The SpMV model is trained on fp32 timings and also chooses for fp64
-    matrices (its features depend only on the sparsity structure).  The
-    held-out matrices timed with fp64 values (spmv_f64_test_db.npz): its
-    choices, recomputed here, are those recorded on the device, close to
-    exhaustive autotuning and faster than the fp64 fixed rule."""
-    if not _have("spmv_f64_test_db.npz", "spmv_f32_mlp.json", "eval.json"):
+def _fp64_regret(orc, model_name):
+    db = TimingDB.load(os.path.join(DIR, "spmv_f64_test_db.npz"))
+    with open(os.path.join(DIR, model_name)) as fh:
+        model = json.load(fh)
+    _, fams, feats, times, sizes = db.arrays()
+    choice, _ = _choices(orc, model, feats)
+    rows = np.arange(times.shape[0])
+    return times[rows, choice] / times.min(axis=1)
+
+
+# (model, its report in eval.json, regret bound, gain over the fp64 fixed rule)
+FP64_MODELS = [("spmv_f32_mlp.json", ("spmv_f64_held_out",), 1.06, 1.05),
+               ("spmv_f64_mlp.json", ("spmv_f64", "held_out"), 1.02, 1.10)]
+
+
+@pytest.mark.parametrize("model_name,report,max_regret,min_gain", FP64_MODELS)
+def test_spmv_model_on_fp64_held_out(orc, model_name, report, max_regret, min_gain):
+    """Held-out matrices timed with fp64 values (spmv_f64_test_db.npz),
+    scored for the fp32-trained SpMV model (its features depend only on the
+    sparsity structure) and for the model trained on the same training
+    matrices timed with fp64 values, the one the library uses for fp64
+    values: the choices, recomputed here, are those recorded on the device,
+    close to exhaustive autotuning and faster than the fp64 fixed rule."""
+    if not _have("spmv_f64_test_db.npz", model_name, "eval.json"):
         pytest.skip("no fp64 evaluation committed")
     db = TimingDB.load(os.path.join(DIR, "spmv_f64_test_db.npz"))
-    with open(os.path.join(DIR, "spmv_f32_mlp.json")) as fh:
+    with open(os.path.join(DIR, model_name)) as fh:
         model = json.load(fh)
     with open(os.path.join(DIR, "eval.json")) as fh:
-        ev = json.load(fh)["spmv_f64_held_out"]
+        ev = json.load(fh)
+    for k in report:
+        ev = ev[k]
     _, fams, feats, times, sizes = db.arrays()
     choice, _ = _choices(orc, model, feats)
     rows = np.arange(times.shape[0])
@@ -107,11 +127,23 @@ def test_spmv_model_on_fp64_held_out(orc):
     t_best = times.min(axis=1)
     regret = float(np.exp(np.mean(np.log(t_sel / t_best))))
     assert abs(regret - ev["geomean_regret"]) <= 2e-3 * ev["geomean_regret"]
-    assert regret < 1.06
+    assert regret < max_regret
     heur = np.asarray([spmv_heuristic_index(int(n), int(n), int(z), item=8,
                                             local=f in LOCAL_FAMILIES)
                        for (n, z), f in zip(sizes, fams)])
     t_heu = times[rows, heur]
     gain = float(np.exp(np.mean(np.log(t_heu / t_sel))))
     assert abs(gain - ev["geomean_speedup_vs_heuristic"]) <= 2e-3 * gain
-    assert gain > 1.05
+    assert gain > min_gain
+
+
+def test_fp64_model_beats_fp32_model_on_fp64_held_out(orc):
+    """On the fp64 held-out timings the fp64-trained model is closer to
+    exhaustive autotuning than the fp32-trained one: lower geometric-mean
+    regret and more matrices within 5% of the fastest variant."""
+    if not _have("spmv_f64_test_db.npz", "spmv_f64_mlp.json", "spmv_f32_mlp.json"):
+        pytest.skip("no fp64 model committed")
+    r32 = _fp64_regret(orc, "spmv_f32_mlp.json")
+    r64 = _fp64_regret(orc, "spmv_f64_mlp.json")
+    assert np.exp(np.mean(np.log(r64))) < np.exp(np.mean(np.log(r32)))
+    assert np.mean(r64 <= 1.05) > np.mean(r32 <= 1.05)
