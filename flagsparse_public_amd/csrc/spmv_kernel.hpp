@@ -1012,21 +1012,16 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS,
   using S = typename TileSumOf<A>::T;
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t + 1 >= p.n_tiles) return;  // the last tile ends at nnz: no row runs past it
-  const int64_t r0 = p.tile_start[t], r1 = p.tile_start[t + 1];
-  if (r1 <= r0 || r1 > p.n_rows) return;
-  if ((int64_t)p.ptr[r1] <= p.tile_entry[t + 1]) return;
   const S* carries = reinterpret_cast<const S*>(p.partials);
-  S acc = carries[2 * t + 1];
   // the following tiles kWalk at a time: their heads and row starts are
-  // loaded together (indices clamped to the last tile), then added in tile
-  // order up to the tile that owns the next row -- the same sum as one tile
-  // at a time, without a dependent load per tile
+  // loaded together (indices clamped to the last tile, so every load is in
+  // bounds), then added in tile order up to the tile that owns the next row
+  // -- the same sum as one tile at a time, without a dependent load per tile
   constexpr int kWalk = 8;
   const int64_t last = p.n_tiles - 1;
-  bool done = false;
-  for (int64_t u0 = t + 1; u0 < p.n_tiles && !done; u0 += kWalk) {
-    S head[kWalk];
-    int64_t ts[kWalk + 1];
+  S head[kWalk];
+  int64_t ts[kWalk + 1];
+  auto load_walk = [&](int64_t u0) {
 #pragma unroll
     for (int k = 0; k < kWalk; ++k) {
       const int64_t u = u0 + k < last ? u0 + k : last;
@@ -1034,6 +1029,19 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS,
       ts[k] = p.tile_start[u];
     }
     ts[kWalk] = p.tile_start[u0 + kWalk < p.n_tiles ? u0 + kWalk : p.n_tiles];
+  };
+  // The first walk depends only on t, so it is loaded with the straddle
+  // test's own operands rather than after them: two dependent round trips
+  // (tile bounds, then the row pointer) instead of three for the common
+  // one-tile case.
+  load_walk(t + 1);
+  S acc = carries[2 * t + 1];
+  const int64_t r0 = p.tile_start[t], r1 = ts[0];
+  const int64_t e1 = p.tile_entry[t + 1];
+  if (r1 <= r0 || r1 > p.n_rows) return;
+  if ((int64_t)p.ptr[r1] <= e1) return;
+  bool done = false;
+  for (int64_t u0 = t + 1;;) {
 #pragma unroll
     for (int k = 0; k < kWalk; ++k) {
       if (done || u0 + k >= p.n_tiles) break;
@@ -1041,6 +1049,9 @@ __global__ __launch_bounds__(kBlock) void spmv_entry_fixup_kernel(SpmvParams<VS,
       const int64_t next = k + 1 < kWalk ? ts[k + 1] : ts[kWalk];
       if (u0 + k == last || next > ts[k]) done = true;
     }
+    u0 += kWalk;
+    if (done || u0 >= p.n_tiles) break;
+    load_walk(u0);
   }
   entry_write(p, r1 - 1, acc);
 }
