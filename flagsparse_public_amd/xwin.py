@@ -27,9 +27,12 @@ Layout (all on the device, built once per matrix by stable sorts):
   gather finds an entry's run with one popcount.
 * ``x_perm[j]`` (uint16): position of CSR entry j inside its tile's slice of
   ``xg``.
-* parts: the gather's workgroups, one per CU, each an equal range of the
-  window-major entries (a range runs through the windows it covers, staging
-  each in turn).
+* parts: the gather's workgroups, about one per CU.  For the window-major
+  xg each part lies inside one window (the parts dealt to the windows in
+  proportion to their entries), so each part stages one window; for the
+  tile-major xg they are equal ranges of the window-major entries cut on
+  64-entry batches, and a range runs through the windows it covers, staging
+  each in turn.
 
 Bytes per entry (fp32): pass 1 reads 2 (wcol) and writes 4 (xg); pass 2 reads
 4 (value) + 4 (xg) + 2 (x_perm).  16 bytes, all in streams, against 12 in the
@@ -114,14 +117,44 @@ def gather_parts(device, window_bytes: int) -> int:
     return cus * max(1, lds // max(window_bytes, 1))
 
 
-def part_ranges(win_start, nnz: int, n_parts: int):
-    """Cut the window-major entries [0, nnz) into n_parts about equal ranges
-    (cuts on multiples of 64, the gather's batch size); a range may run
-    through several windows.  Returns host lists (part_window, part_entry):
-    the window holding each range's first entry, and the range starts plus
-    nnz at the end."""
+def part_ranges(win_start, nnz: int, n_parts: int, window_aligned: bool = False):
+    """Cut the window-major entries [0, nnz) into about n_parts ranges.
+    Returns host lists (part_window, part_entry): the window holding each
+    range's first entry, and the range starts plus nnz at the end.
+
+    Default: n_parts about equal ranges cut on multiples of 64 (the
+    tile-major gather's batch size); a range may run through several
+    windows and stages each.  ``window_aligned`` (the window-major gather,
+    which takes a range starting anywhere): when the non-empty windows are
+    no more than n_parts, the parts are dealt to them in proportion to their
+    entries (at least one each, largest remainders first) and cut inside each
+    window on multiples of 64 -- one window staged per part, n_parts
+    stagings in all instead of n_parts plus one per window boundary that
+    falls inside a range."""
     import bisect
     ws = [int(v) for v in win_start]
+    if window_aligned and nnz > 0:
+        counts = [ws[w + 1] - ws[w] for w in range(len(ws) - 1)]
+        full = [w for w, c in enumerate(counts) if c > 0]
+        if len(full) <= n_parts:
+            share = [counts[w] * n_parts / nnz for w in full]
+            alloc = [max(1, int(s)) for s in share]
+            spare = n_parts - sum(alloc)
+            if spare > 0:
+                by_rest = sorted(range(len(full)), key=lambda i: int(share[i]) - share[i])
+                for i in by_rest[:spare]:
+                    alloc[i] += 1
+            pw, cuts = [], []
+            for w, a in zip(full, alloc):
+                lo, hi = ws[w], ws[w + 1]
+                pw.append(w)
+                cuts.append(lo)
+                for j in range(1, a):
+                    c = -(-(lo + (hi - lo) * j // a) // 64) * 64
+                    if cuts[-1] < c < hi:
+                        pw.append(w)
+                        cuts.append(c)
+            return pw, cuts + [nnz]
     cuts = [0]
     for i in range(1, max(int(n_parts), 1)):
         c = -(-(nnz * i // n_parts) // 64) * 64
@@ -254,7 +287,8 @@ class XWinCsr:
         del win, order1
         self.window_entry = ws.contiguous()
         item = torch.empty(0, dtype=self.value_dtype).element_size()
-        pw, pe = part_ranges(ws.cpu().tolist(), nnz, gather_parts(dev, W * item))
+        pw, pe = part_ranges(ws.cpu().tolist(), nnz, gather_parts(dev, W * item),
+                             window_aligned=self.order == "window")
         self.part_window = torch.tensor(pw, dtype=torch.int32, device=dev)
         self.part_entry = torch.tensor(pe, dtype=torch.int64, device=dev)
         self.n_parts = len(pw)

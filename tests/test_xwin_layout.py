@@ -20,6 +20,28 @@ from flagsparse_public_amd import synthetic
 from flagsparse_public_amd.xwin import XWinCsr, part_ranges
 
 
+def _part_cover(pw, pe, ws, nnz):
+    """The gather kernels' walk over parts (csrc/xwin.hip): part p starts in
+    window pw[p] at entry pe[p] and runs through windows until pe[p + 1].
+    Returns, per entry, how many parts visited it and the window it was
+    staged from, and the number of window stagings."""
+    seen = np.zeros(nnz, np.int64)
+    staged_win = np.full(nnz, -1, np.int64)
+    stagings = 0
+    n_windows = len(ws) - 1
+    for p in range(len(pw)):
+        k0, kend, w = int(pe[p]), int(pe[p + 1]), int(pw[p])
+        while k0 < kend and w < n_windows:
+            k1 = min(int(ws[w + 1]), kend)
+            if k1 > k0:
+                stagings += 1
+                seen[k0:k1] += 1
+                staged_win[k0:k1] = w
+                k0 = k1
+            w += 1
+    return seen, staged_win, stagings
+
+
 def _replay(xw, x):
     nnz = xw.nnz
     wcol = xw.wcol.numpy().view(np.uint16).astype(np.int64)
@@ -32,7 +54,10 @@ def _replay(xw, x):
     assert pe[0] == 0 and pe[-1] == nnz and np.all(np.diff(pe) > 0)
     assert ws[0] == 0 and ws[-1] == nnz and np.all(np.diff(ws) >= 0)
     assert np.all((ws[pw] <= pe[:-1]) & (pe[:-1] < ws[pw + 1]))  # first window of each part
+    seen, staged_win, _ = _part_cover(pw, pe, ws, nnz)
     k = np.arange(nnz, dtype=np.int64)
+    assert np.all(seen == 1)  # every entry gathered by exactly one part
+    assert np.array_equal(staged_win, np.searchsorted(ws, k, side="right") - 1)
     lane = (k % 64).astype(np.uint64)
     below = (np.uint64(2) << lane) - np.uint64(2)  # lane 63: 2 << 63 wraps to 0
     r = br[k // 64] + np.bitwise_count(bm[k // 64] & below).astype(np.int64)
@@ -147,3 +172,44 @@ def test_part_ranges_cover_windows():
     assert 1 not in pw  # the empty window holds no entry
     pw, pe = part_ranges([0, 100], 100, 16)  # fewer batches than parts
     assert pe == [0, 64, 100] and pw == [0, 0]
+
+
+def test_part_ranges_window_aligned():
+    """Window-major gather parts: each inside one window, dealt in
+    proportion to the windows' entries, so each part stages one window --
+    fewer stagings than equal ranges, which stage one more window per
+    boundary they cross; every entry still gathered once from its window."""
+    rng = np.random.default_rng(3)
+    counts = rng.integers(30_000, 50_000, size=23)
+    counts[[4, 11]] = 0  # empty windows get no part
+    counts[7] = 1_000  # a small window gets one part
+    ws = np.concatenate([[0], np.cumsum(counts)]).tolist()
+    nnz = ws[-1]
+    n_parts = 64
+    pw, pe = part_ranges(ws, nnz, n_parts, window_aligned=True)
+    pw, pe = np.asarray(pw), np.asarray(pe)
+    assert pe[0] == 0 and pe[-1] == nnz and np.all(np.diff(pe) > 0)
+    assert len(pw) == n_parts  # every share >= 1: exactly n_parts parts
+    ws_a = np.asarray(ws)
+    assert np.all((ws_a[pw] <= pe[:-1]) & (pe[1:] <= ws_a[pw + 1]))  # inside one window
+    inner = pe[:-1][pe[:-1] != ws_a[pw]]
+    assert np.all(inner % 64 == 0)  # cuts inside a window on 64-entry batches
+    for w in range(len(counts)):
+        sizes = np.diff(pe)[pw == w]
+        if counts[w] == 0:
+            assert sizes.size == 0
+            continue
+        # equal shares, each cut rounded up to a batch boundary
+        assert sizes.sum() == counts[w] and sizes.max() - sizes.min() <= 128
+        # parts in proportion to the window's entries (largest remainders)
+        share = counts[w] * n_parts / nnz
+        assert int(share) <= sizes.size <= max(1, int(share) + 1)
+    seen, staged_win, stagings = _part_cover(pw, pe, ws, nnz)
+    k = np.arange(nnz)
+    assert np.all(seen == 1)
+    assert np.array_equal(staged_win, np.searchsorted(ws_a, k, side="right") - 1)
+    assert stagings == n_parts
+    upw, upe = part_ranges(ws, nnz, n_parts)
+    assert _part_cover(upw, upe, ws, nnz)[2] > stagings
+    # more non-empty windows than parts: the equal ranges
+    assert part_ranges(ws, nnz, 8, window_aligned=True) == part_ranges(ws, nnz, 8)
