@@ -175,8 +175,9 @@ def test_part_ranges_cover_windows():
 
 
 def test_part_ranges_window_aligned():
-    """Window-major gather parts: each inside one window, dealt in
-    proportion to the windows' entries, so each part stages one window --
+    """Window-major gather parts with at least two parts per non-empty
+    window: each inside one window, dealt in proportion to the windows'
+    entries, so each part stages one window --
     fewer stagings than equal ranges, which stage one more window per
     boundary they cross; every entry still gathered once from its window."""
     rng = np.random.default_rng(3)
@@ -211,5 +212,7 @@ def test_part_ranges_window_aligned():
     assert stagings == n_parts
     upw, upe = part_ranges(ws, nnz, n_parts)
     assert _part_cover(upw, upe, ws, nnz)[2] > stagings
-    # more non-empty windows than parts: the equal ranges
+    # fewer than two parts per non-empty window (21 of them): the equal ranges
     assert part_ranges(ws, nnz, 8, window_aligned=True) == part_ranges(ws, nnz, 8)
+    assert part_ranges(ws, nnz, 41, window_aligned=True) == part_ranges(ws, nnz, 41)
+    assert part_ranges(ws, nnz, 42, window_aligned=True) != part_ranges(ws, nnz, 42)
