@@ -522,3 +522,30 @@ def test_exhaustive_schedule_with_int64_columns(orc, dtype):
     cond = orc.spmv_abs(ip, cols, vals, x)
     # int64 columns take the default path (fp64 accumulation for fp32), as in the reference
     assert row_rel_err(host64(y), y64, cond).max() <= (1e-6 if dtype == torch.float32 else 1e-12)
+
+
+def test_stream_handle_follows_current_stream():
+    """The library launches on torch's current stream: the raw handle it
+    passes through the C-ABI is the one torch.cuda.stream selects, for a
+    device given as torch.device, "cuda" or an index, and a prepared SpMV
+    issued on a side stream (default priority) equals the one on the default
+    stream, bit for bit."""
+    from flagsparse_public_amd import _lib
+    dev = torch.device("cuda", 0)
+    cur = torch.cuda.current_stream(dev).cuda_stream
+    assert _lib.stream_handle(dev) == cur
+    assert _lib.stream_handle("cuda") == cur and _lib.stream_handle(0) == cur
+    side = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(side):
+        assert _lib.stream_handle(dev) == side.cuda_stream != cur
+    ip, cols, vals = synthetic.spmv_16k()
+    n = ip.size - 1
+    p = prepare_spmv_csr(*(torch.from_numpy(a).to(dev) for a in (vals, cols, ip)), (n, n))
+    x = torch.from_numpy(synthetic.dense_uniform(11, n)).to(dev)
+    y0 = flagsparse_spmv_csr(x=x, prepared=p, use_opt=True)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        y1 = flagsparse_spmv_csr(x=x, prepared=p, use_opt=True)
+    torch.cuda.current_stream(dev).wait_stream(side)
+    torch.cuda.synchronize()
+    assert torch.equal(y0, y1)
