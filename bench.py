@@ -864,12 +864,12 @@ def main(argv=None):
     # the dominant kernel alone: the gather pass of the x-window layout, the
     # compact-row SpMV of the sliced layout, or the whole SpMV otherwise
     from flagsparse_public_amd import _lib
-    sliced = prepared._plans.get(("sliced", sch.key(), _lib.FS_ACC_NATIVE if use_opt
-                                  else _lib.FS_ACC_WIDE))
-    xw = prepared._plans.get(("xwin", sch.key()))
+    from flagsparse_public_amd.sparse_operations.spmv_csr import xwin_plan_key
+    acc = _lib.FS_ACC_NATIVE if use_opt else _lib.FS_ACC_WIDE
+    sliced = prepared._plans.get(("sliced", sch.key(), acc))
+    xw = prepared._plans.get(xwin_plan_key(prepared, sch, acc)[0])
     kernels = None
     if xw is not None:
-        acc = _lib.FS_ACC_NATIVE if use_opt else _lib.FS_ACC_WIDE
         pst = xw.plan(acc).struct(1)
         yk = torch.empty(n, dtype=d_vals.dtype, device=dev)
         n_batches = -(-nnz // 64)

@@ -14,6 +14,7 @@ import torch
 
 from flagsparse_public_amd import _lib, flagsparse_spmv_csr, prepare_spmv_csr, synthetic
 from flagsparse_public_amd.rowsplit import RowSplitPlan, Schedule
+from flagsparse_public_amd.sparse_operations.spmv_csr import XWIN_WIDE_TILE
 from flagsparse_public_amd.xwin import XWinCsr
 from tests._util import host64, row_rel_err, to_dev, ulp_distance_f32
 
@@ -70,7 +71,10 @@ def test_xwin_equals_entry_tiles(orc, kind, dtype, use_opt, T, window):
           flush=True)
     x = synthetic.dense_uniform(5, n_cols, dtype=vals.dtype)
     args = [to_dev(vals), to_dev(cols), to_dev(ip)]
-    ref = prepare_spmv_csr(*args, (n, n_cols), schedule=Schedule(tile_nnz=T, tile_mode=1))
+    # the default fp32 path runs x-window layouts with tiles of at most
+    # XWIN_WIDE_TILE entries: the entry tiles it equals are cut the same way
+    T_ref = min(T, XWIN_WIDE_TILE) if dtype == torch.float32 and not use_opt else T
+    ref = prepare_spmv_csr(*args, (n, n_cols), schedule=Schedule(tile_nnz=T_ref, tile_mode=1))
     xw = prepare_spmv_csr(*args, (n, n_cols),
                           schedule=Schedule(tile_nnz=T, tile_mode=1, x_window=window))
     dx = to_dev(x)
@@ -79,6 +83,8 @@ def test_xwin_equals_entry_tiles(orc, kind, dtype, use_opt, T, window):
     assert torch.equal(y, y_ref)
     y2 = flagsparse_spmv_csr(x=dx, prepared=xw, use_opt=use_opt)  # reuses the layout
     assert torch.equal(y2, y)
+    layouts = [v for k, v in xw._plans.items() if isinstance(k, tuple) and k[0] == "xwin"]
+    assert len(layouts) == 1 and layouts[0].tile_nnz == T_ref
     y64, cond = _oracle(orc, ip, cols, vals, x)
     _check(y, y64, cond, dtype, use_opt)
 
