@@ -672,7 +672,9 @@ def sub_unprepared(d_spmv, prepared_us, reps=20):
     the 16K-row one: wall time per call (each call runs the validation pass
     and its host read, as the reference's does), beside the prepared step and
     the validation pass alone.  The first call prepares; later calls on the
-    same tensors reuse it."""
+    same tensors reuse it.  Timed with use_opt=True (the opt path, as the
+    prepared step), and again with the reference's default use_opt=False
+    (the default path: fp32 summed in fp64)."""
     import numpy as np
     import torch
     from flagsparse_public_amd import _lib, flagsparse_spmv_csr, prepare_spmv_csr, synthetic
@@ -704,12 +706,16 @@ def sub_unprepared(d_spmv, prepared_us, reps=20):
         call = lambda: flagsparse_spmv_csr(vals_d, cols_d, ip_d, x_d, (n, n), use_opt=True)
         first = wall(call, 1)
         again = wall(call, reps)
+        call_default = lambda: flagsparse_spmv_csr(vals_d, cols_d, ip_d, x_d, (n, n))
+        call_default()  # the default path's own layout, built once
+        again_default = wall(call_default, reps)
         valid = wall(lambda: _prepare_spmv_csr_matrix(vals_d, cols_d, ip_d, (n, n),
                                                       _lengths=False), reps)
         csr = _lib.make_csr(vals_d, cols_d, ip_d, n, n)
         kern = _event_time(lambda: _lib.analyze_device(csr, dev), reps,
                            torch.cuda.current_stream(dev))
         out[name] = {"first_call_us": round(first, 1), "repeat_call_us": round(again, 1),
+                     "repeat_call_default_path_us": round(again_default, 1),
                      "prepared_step_us": round(step_us, 2), "validation_pass_us": round(valid, 1),
                      "validation_kernel_us": round(kern, 1),
                      "repeat_over_step_plus_validation": round(again / (step_us + valid), 3)}
